@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the SWARM data back-end GPU stage on one MI355X.
+
+Workload (the plain FP32 GPU pipeline at full rate): 256 consecutive
+39-snapshot blocks x 2 polarisations x 16384 channels = 2 groups of 39
+B-frames = 159,744 B-engine-over-VDIF packets of 1056 bytes (reference
+geometry: 128 snapshots x 8 channels x 2 pols per packet), circular Gaussian
+2-bit spectra plus a -20 dB tone, seed-shuffled arrival order, 0.1% of the
+packets dropped.  One step = both groups from device-resident packets to
+device-resident VDIF frames (32-byte header + 8192-byte payload): unpack,
+reorder, zero-fill, 39 x C2R 32768, R2C 1,277,952, zero-pad C2R 2,097,152,
+per-block FP64 sigma, 2-bit quantise, pack, headers.
+
+Metric: input Gsample/s = block-pols x 1,277,952 real SWARM-rate samples per
+second.  Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("sustained input Gsample/s per MI355X (2-bit in→2-bit VDIF out); % HBM roofline")
+SAMPLES_PER_BP = 1277952
+HBM_PEAK = 8.0e12
+
+# Algorithmic HBM bytes per unit of work of each kernel (unit: one block-pol
+# for the resampler kernels, one group for the unpack kernels): inputs read
+# once plus outputs written once.
+ALG_BYTES = {
+    "k_packet_map": 79872 * 32,                          # header sector per packet
+    "k_reorder": 79872 * 1024 + 2 * 4992 * 8192,         # payloads in, 4-bit rows out
+    "k_stage_a": 39 * 16384 // 2 + 39 * 16384 * 8,       # 4-bit rows in, V out
+    "k_stage_b": 2 * 39 * 16384 * 8,                     # V in, packed R2C out
+    "k_stage_c1": 39 * 16384 * 8 + (1 << 20) * 8,        # packed R2C in, first C2R pass out
+    "k_stage_c2": (1 << 20) * 8 + (1 << 20),             # second pass in, 4-bit codes out
+    "k_pack": (1 << 20) + 64 * 8192,                     # codes in, payloads out
+    "k_headers": 64 * 32,
+    "k_stats": 513 * 16,
+    "k_fill_i32": 79872 * 4,
+}
+B_ALG_PER_BP = 48304512          # stage-boundary algorithmic bytes per block-pol
+COMPULSORY_PER_BP = 329472 + 526336
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(groups, n_blocks=8):
+    """The oracle's CPU chain (numpy/scipy FP64) on a bounded sample: unpack of
+    one whole group, then resample + sigma + quantise + pack of n_blocks x 2
+    pols.  Rate counts the unpack share of the sampled block-pols."""
+    import numpy as np
+    import scipy
+    from oracle import sdbe_oracle as O
+    workers = os.cpu_count() or 1
+    pk, b0 = groups[0]
+    t0 = time.perf_counter()
+    un = O.vdif_to_beng(pk, b0)
+    t_unpack = time.perf_counter() - t0
+    blocks = list(range(n_blocks))
+    X = np.stack([O.block_spectra(un.beng[pol], blk) for blk in blocks for pol in (0, 1)])
+    t0 = time.perf_counter()
+    x = O.cpu_resample_batch(X, workers)
+    for i in range(x.shape[0]):
+        mean, sig = O.block_stats(x[i])
+        O.pack_codes(O.quantize2bit(x[i], sig, mean))
+    O.stamp_headers(n_blocks * 64, O.StreamState())
+    t_chain = time.perf_counter() - t0
+    n_bp = 2 * n_blocks
+    t = t_unpack * n_bp / 256.0 + t_chain
+    return {
+        "value": n_bp * SAMPLES_PER_BP / t / 1e9,
+        "unit": "Gsample/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": ("oracle CPU chain (numpy unpack of one 79,872-packet group, 1 thread: %.2f s; "
+                   "scipy.fft FP64 C2R/R2C/C2R + sigma + quantise + pack of %d block-pols with "
+                   "workers=%d: %.2f s; numpy %s, scipy %s, FFTW absent)"
+                   % (t_unpack, n_bp, workers, t_chain, np.__version__, scipy.__version__)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--policy", default="per_block", choices=["per_block", "reference"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=2026)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: every GPU box here has one GPU, and blocks are "
+              "independent (N GPUs = N disjoint block ranges, no exchange); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from sdbe_amd import _lib as L
+    from sdbe_amd import synth
+    from sdbe_amd.stage import SdbeStage
+
+    log("[bench] generating 2 groups of packets (seed %d)" % args.seed)
+    groups, _ = synth.make_group_packets(seed=args.seed, n_groups=2, drop_frac=0.001)
+    n_pkts = sum(pk.shape[0] for pk, _ in groups)
+    policy = L.THRESH_PER_BLOCK if args.policy == "per_block" else L.THRESH_REFERENCE
+    st = SdbeStage(policy=policy, out_mode=L.OUT_FRAMES, chunk=args.chunk, station="Sm")
+    st.set_state(secs=1000, df=0, ref_epoch=30, psn=1)
+    dpk = [st.to_device_packets(pk) for pk, _ in groups]
+    outs = [st.group_out() for _ in groups]
+    torch.cuda.synchronize()
+
+    def step():
+        for g, (pk, b0) in enumerate(groups):
+            st.process_group(dpk[g], pk.shape[0], b0, out=outs[g])
+
+    log("[bench] warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    st.profile(True)
+    st.profile_read()
+    log("[bench] timing %d steps" % args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = st.profile_read()
+    st.profile(False)
+    stats = st.stats()
+
+    n_bp = 2 * 2 * 128
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = n_bp * SAMPLES_PER_BP / (ms_step * 1e-3) / 1e9
+    bp_per_s = n_bp / (ms_step * 1e-3)
+
+    kernels = {}
+    for name, (launches, total_ms, units) in prof.items():
+        ab = ALG_BYTES.get(name)
+        kernels[name] = {
+            "launches": launches,
+            "avg_us": round(1e3 * total_ms / launches, 3),
+            "ms_per_step": round(total_ms / args.steps, 4),
+            "alg_GBps": round(ab * units / (total_ms * 1e-3) / 1e9, 1) if ab and total_ms > 0 else None,
+        }
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dname, (dl, dms, du) = dom
+    d_bytes_per_launch = ALG_BYTES[dname] * du / dl
+    d_avg_s = dms * 1e-3 / dl
+    achieved = d_bytes_per_launch / d_avg_s / 1e9
+
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gsample/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain FP32 pipeline at full rate: 256 blocks x 2 pols x 16384 ch "
+                        "(2 groups of 39 B-frames), shuffled packets, 0.1% dropped, VDIF frames out",
+            "block_pols_per_step": n_bp,
+            "packets_per_step": int(n_pkts),
+            "threshold_policy": args.policy,
+            "chunk_blockpols": args.chunk,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dname,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+            "traffic": None,
+        },
+        "pipeline": {
+            "block_pols_per_s": round(bp_per_s, 1),
+            "b_alg_frac": round(B_ALG_PER_BP * bp_per_s / HBM_PEAK, 4),
+            "compulsory_frac": round(COMPULSORY_PER_BP * bp_per_s / HBM_PEAK, 5),
+            "realtime_pol_streams": round(value / 2.496, 2),
+        },
+        "kernels": kernels,
+        "stage_stats": {k: stats[k] for k in ("n_unique", "n_missing", "n_duplicate", "n_invalid")},
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        rec["cpu_baseline"] = cpu_baseline(groups)
+    print(json.dumps(rec))
+    st.close()
+
+
+if __name__ == "__main__":
+    main()

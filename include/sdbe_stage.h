@@ -1,0 +1,206 @@
+/*
+ * sdbe_stage.h -- C-ABI of the MI355X (gfx950) SWARM data back-end GPU stage.
+ *
+ * The stage takes one group of SWARM B-engine-over-VDIF packets (39 B-frames,
+ * up to 79,872 packets of 1056 bytes, any order, some missing) resident in
+ * device memory and produces 2-bit VDIF output for both polarisations:
+ * 128 blocks of 39 snapshots each, resampled 39 -> 64 (2496 -> 4096 Msps),
+ * 64 frames of 8192 payload bytes per block and polarisation.
+ *
+ * Plain pointers and sizes only; device pointers are HIP device addresses,
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ * Every call returns SDBE_OK (0) or a negative SDBE_ERR_* code and never
+ * throws.  Calls on one handle must not overlap.
+ *
+ * Reference interfaces replaced (paths relative to the Smithsonian/sdbe tree):
+ *   sdbe_create / sdbe_destroy  -> aphids_resampler_init / _destroy
+ *                                  (software/aphids/vdif_inout_gpu_thread.cu:184-347, 712-743)
+ *   sdbe_process_group          -> the per-slot body of run_method: memset +
+ *                                  vdif_to_beng + 128 x [inflate, C2R, R2C, C2R,
+ *                                  thresholds, quantize2bit]
+ *                                  (vdif_inout_gpu_thread.cu:894-1008), plus the
+ *                                  output thread's per-packet header stamping
+ *                                  (software/aphids/vdif_out_net_thread.c:294-317)
+ *   sdbe_unpack_group_int8      -> vdif_to_beng with its zero fill
+ *                                  (vdif_inout_gpu_thread.cu:404-507, 897-906)
+ *   sdbe_process_spectra        -> fft_batched + quantize on already-reordered
+ *                                  spectra (software/prototyping/cuda/resample.py:67-146,
+ *                                  vdif_inout_gpu_thread.cu:923-1008)
+ *   sdbe_resample_spectra_f32   -> the three cuFFT stages alone
+ *                                  (vdif_inout_gpu_thread.cu:589-709;
+ *                                  resample.py:109-123), normalised like numpy
+ *   sdbe_stream_state_from_template -> fill_vdif_header_template + the output
+ *                                  thread's start alignment
+ *                                  (software/aphids/vdif_in_databuf.c:193-220,
+ *                                  vdif_out_net_thread.c:203-237)
+ */
+#ifndef SDBE_STAGE_H
+#define SDBE_STAGE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------ */
+#define SDBE_OK              0
+#define SDBE_ERR_ARG        -1   /* bad argument (NULL, size out of range)   */
+#define SDBE_ERR_HIP        -2   /* a HIP runtime call failed                */
+#define SDBE_ERR_NOMEM      -3   /* device or host allocation failed         */
+#define SDBE_ERR_CAPACITY   -4   /* request exceeds the handle's capacity    */
+#define SDBE_ERR_NODEV      -5   /* no such HIP device                       */
+#define SDBE_ERR_STATE      -6   /* call not valid in the current state      */
+
+/* ---- geometry (6/11 SWARM rate, 64/39 resampling) ----------------------- */
+#define SDBE_CHANNELS            16384  /* complex channels per snapshot     */
+#define SDBE_SNAPSHOTS_PER_FRAME 128
+#define SDBE_FRAMES_PER_GROUP    39     /* B-frames per group = snapshots per block */
+#define SDBE_PKTS_PER_FRAME      2048
+#define SDBE_PKT_BYTES           1056
+#define SDBE_BLOCKS_PER_GROUP    128
+#define SDBE_SAMPLES_PER_BLOCK   2097152 /* output samples per block and pol */
+#define SDBE_OUT_FRAMES_PER_BLOCK 64
+#define SDBE_OUT_PAYLOAD_BYTES   8192
+#define SDBE_OUT_HEADER_BYTES    32
+#define SDBE_OUT_FRAMES_PER_GROUP (SDBE_BLOCKS_PER_GROUP * SDBE_OUT_FRAMES_PER_BLOCK)
+
+/* ---- threshold policy --------------------------------------------------- */
+/* REFERENCE: mean and ddof=1 sigma of the first block of polarisation 0 are
+ *   latched for the whole stream and used for both polarisations, recomputed
+ *   while the latched sigma is 0 (vdif_inout_gpu_thread.cu:954-1005).
+ * PER_BLOCK: every block and polarisation uses its own mean and sigma.
+ * FIXED: caller-given threshold and offset.                                */
+#define SDBE_THRESH_REFERENCE 0
+#define SDBE_THRESH_PER_BLOCK 1
+#define SDBE_THRESH_FIXED     2
+
+/* ---- output layout ------------------------------------------------------ */
+/* PAYLOAD: 8192-byte payloads back to back per polarisation, the layout of
+ *   vdif_out_data_group_t (software/aphids/vdif_out_databuf.h:110-116).
+ * FRAMES: complete VDIF frames (32-byte header + payload) back to back per
+ *   polarisation, the layout of vdif_out_packet_block_t (vdif_out_databuf.h:66-98). */
+#define SDBE_OUT_PAYLOAD 0
+#define SDBE_OUT_FRAMES  1
+
+typedef struct sdbe_config {
+    int32_t thresh_policy;      /* SDBE_THRESH_*                                */
+    float   fixed_threshold;    /* used with SDBE_THRESH_FIXED                  */
+    float   fixed_offset;
+    int32_t out_mode;           /* SDBE_OUT_*                                   */
+    int32_t chunk_blockpols;    /* block-pols per pipeline chunk (even, 2..256) */
+    int32_t max_spectra_blockpols; /* capacity of sdbe_process_spectra (>= 1)   */
+    uint16_t station_id;        /* VDIF w3.stationID (2 ASCII chars, "XX")      */
+    uint8_t  bdc_sideband;      /* VDIF w4.bdc_sideband                        */
+    uint8_t  rx_sideband;       /* VDIF w4.rx_sideband                         */
+    int32_t reserved[8];
+} sdbe_config_t;
+
+/* VDIF output time state for the next frame to be produced; both
+ * polarisations share it.  df wraps at 125000 frames per second.           */
+typedef struct sdbe_stream_state {
+    uint32_t secs;              /* w0.secs_inre                                 */
+    uint32_t df;                /* w1.df_num_insec                              */
+    uint32_t ref_epoch;         /* w1.ref_epoch                                 */
+    uint32_t reserved;
+    uint64_t psn;               /* edh_psn of the next frame                    */
+} sdbe_stream_state_t;
+
+/* Counters of the last completed call (filled by sdbe_get_stats).          */
+typedef struct sdbe_stats {
+    int32_t n_packets;          /* packets handed in                            */
+    int32_t n_invalid;          /* w0.invalid set: ignored                      */
+    int32_t n_out_of_range;     /* B-count outside the group or fid >= 8        */
+    int32_t n_duplicate;        /* repeated (frame, cid, fid): first one used   */
+    int32_t n_unique;           /* (frame, cid, fid) slots filled               */
+    int32_t n_missing;          /* slots zero-filled                            */
+    int32_t n_blockpols;        /* block-pols resampled                         */
+    int32_t reserved;
+    double  latched_threshold;  /* REFERENCE policy latch (0 if not latched)    */
+    double  latched_offset;
+} sdbe_stats_t;
+
+typedef struct sdbe_handle sdbe_handle_t;
+
+/* Fill `cfg` with the reference defaults (REFERENCE policy, payload output,
+ * station "XX", chunk of 32 block-pols).                                    */
+int sdbe_default_config(sdbe_config_t *cfg);
+
+/* Allocate plans, twiddle tables and scratch on HIP device `device`.        */
+int sdbe_create(const sdbe_config_t *cfg, int device, sdbe_handle_t **out);
+void sdbe_destroy(sdbe_handle_t *h);
+
+const char *sdbe_error_string(int code);
+/* The last HIP error code seen by the handle (0 if none).                   */
+int sdbe_last_hip_error(const sdbe_handle_t *h);
+
+int sdbe_set_stream_state(sdbe_handle_t *h, const sdbe_stream_state_t *st);
+int sdbe_get_stream_state(const sdbe_handle_t *h, sdbe_stream_state_t *st);
+
+/* Derive the stream state from the first group's 32-byte header template the
+ * way the output thread does on its first valid template; also returns the
+ * sample alignment it implies (int32 words to skip, frames to skip).        */
+int sdbe_stream_state_from_template(const void *template_header32,
+                                    int32_t n_words_per_chan,
+                                    sdbe_stream_state_t *st,
+                                    int32_t *skip_int32, int32_t *skip_frames);
+
+/* Process one group.  d_pkts: n_pkts packets of 1056 bytes (device).
+ * b_zero: B-count of the group's first frame (bfc[0].b).  d_out: output,
+ * polarisation p at d_out + p * out_pol_stride_bytes, 8192 frames each of
+ * 8192 (PAYLOAD) or 8224 (FRAMES) bytes.  d_bitmap (optional, device):
+ * uint8 [39][256], bit f of [frame][cid] set when packet (frame, cid, f)
+ * arrived (the cc[] flags of vdif_in_databuf.h:112-130).                   */
+int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
+                       uint64_t b_zero, void *d_out, int64_t out_pol_stride_bytes,
+                       uint8_t *d_bitmap, void *stream);
+
+/* Unpack one group into the reference's int8 layout: d_beng0/d_beng1 are
+ * int8 [39][128][16384][2 = (im, re)] (163,577,856 bytes each), missing
+ * packets zero-filled; d_bitmap optional as above.                          */
+int sdbe_unpack_group_int8(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
+                           uint64_t b_zero, int8_t *d_beng0, int8_t *d_beng1,
+                           uint8_t *d_bitmap, void *stream);
+
+/* Resample + quantise + pack n_bp independent block-pols given as int8
+ * spectra d_beng [n_bp][39][16384][2 = (im, re)].  Output of block-pol i is
+ * 64 frames at d_out + i * 64 * frame_bytes.  Threshold policy treats every
+ * block-pol as polarisation 0 of consecutive blocks.                        */
+int sdbe_process_spectra(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp,
+                         void *d_out, void *stream);
+
+/* Resample only: d_samples [n_bp][2097152] float, normalised like numpy's
+ * irfft(pad(rfft(concat(irfft(X))))) (the reference's cuFFT output divided by
+ * 32768 * 2097152); d_stats (optional) [n_bp][2] double = (mean, ddof=1 sigma)
+ * of each block-pol's samples.                                              */
+int sdbe_resample_spectra_f32(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp,
+                              float *d_samples, double *d_stats, void *stream);
+
+/* Synchronise the handle's last stream and copy the counters out.          */
+int sdbe_get_stats(sdbe_handle_t *h, sdbe_stats_t *out);
+
+/* Per-kernel timing for benchmarks (no reference counterpart; the reference
+ * times cuFFT calls with cudaEvents under GPU_DEBUG,
+ * vdif_inout_gpu_thread.cu:593-622).  When enabled, every kernel launch is
+ * bracketed by two HIP events recorded on its own stream.  sdbe_profile_read
+ * synchronises on the last event, returns one record per kernel kind that ran
+ * (launch count, summed milliseconds, work units processed: block-pols for
+ * the resampler kernels, groups for the unpack kernels) and clears them.   */
+typedef struct sdbe_kernel_time {
+    char    name[32];
+    int32_t launches;
+    int32_t reserved;
+    double  total_ms;
+    double  units;
+} sdbe_kernel_time_t;
+
+int sdbe_profile_enable(sdbe_handle_t *h, int enable);
+int sdbe_profile_read(sdbe_handle_t *h, sdbe_kernel_time_t *out, int32_t max_records,
+                      int32_t *n_records);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDBE_STAGE_H */

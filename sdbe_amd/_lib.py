@@ -1,0 +1,144 @@
+"""ctypes binding of include/sdbe_stage.h (libsdbe.so, built in-tree).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdbe.so")
+
+SDBE_OK = 0
+SDBE_ERR_ARG = -1
+SDBE_ERR_HIP = -2
+SDBE_ERR_NOMEM = -3
+SDBE_ERR_CAPACITY = -4
+SDBE_ERR_NODEV = -5
+SDBE_ERR_STATE = -6
+
+THRESH_REFERENCE = 0
+THRESH_PER_BLOCK = 1
+THRESH_FIXED = 2
+OUT_PAYLOAD = 0
+OUT_FRAMES = 1
+
+CHANNELS = 16384
+FRAMES_PER_GROUP = 39
+BLOCKS_PER_GROUP = 128
+SAMPLES_PER_BLOCK = 2097152
+OUT_FRAMES_PER_BLOCK = 64
+OUT_PAYLOAD_BYTES = 8192
+OUT_HEADER_BYTES = 32
+OUT_FRAMES_PER_GROUP = BLOCKS_PER_GROUP * OUT_FRAMES_PER_BLOCK
+PKT_BYTES = 1056
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("thresh_policy", C.c_int32),
+        ("fixed_threshold", C.c_float),
+        ("fixed_offset", C.c_float),
+        ("out_mode", C.c_int32),
+        ("chunk_blockpols", C.c_int32),
+        ("max_spectra_blockpols", C.c_int32),
+        ("station_id", C.c_uint16),
+        ("bdc_sideband", C.c_uint8),
+        ("rx_sideband", C.c_uint8),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class StreamState(C.Structure):
+    _fields_ = [
+        ("secs", C.c_uint32),
+        ("df", C.c_uint32),
+        ("ref_epoch", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("psn", C.c_uint64),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_packets", C.c_int32),
+        ("n_invalid", C.c_int32),
+        ("n_out_of_range", C.c_int32),
+        ("n_duplicate", C.c_int32),
+        ("n_unique", C.c_int32),
+        ("n_missing", C.c_int32),
+        ("n_blockpols", C.c_int32),
+        ("reserved", C.c_int32),
+        ("latched_threshold", C.c_double),
+        ("latched_offset", C.c_double),
+    ]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 32),
+        ("launches", C.c_int32),
+        ("reserved", C.c_int32),
+        ("total_ms", C.c_double),
+        ("units", C.c_double),
+    ]
+
+
+# (name, restype, argtypes) of every symbol declared in include/sdbe_stage.h
+PROTOTYPES = [
+    ("sdbe_default_config", C.c_int, [C.POINTER(Config)]),
+    ("sdbe_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
+    ("sdbe_destroy", None, [C.c_void_p]),
+    ("sdbe_error_string", C.c_char_p, [C.c_int]),
+    ("sdbe_last_hip_error", C.c_int, [C.c_void_p]),
+    ("sdbe_set_stream_state", C.c_int, [C.c_void_p, C.POINTER(StreamState)]),
+    ("sdbe_get_stream_state", C.c_int, [C.c_void_p, C.POINTER(StreamState)]),
+    ("sdbe_stream_state_from_template", C.c_int,
+     [C.c_void_p, C.c_int32, C.POINTER(StreamState), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("sdbe_process_group", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    ("sdbe_unpack_group_int8", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("sdbe_process_spectra", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("sdbe_resample_spectra_f32", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("sdbe_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("sdbe_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
+    ("sdbe_profile_read", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int32, C.POINTER(C.c_int32)]),
+]
+
+_lib = None
+
+
+def load():
+    """Load libsdbe.so and bind every prototype; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsdbe.so is not built (run `python -m sdbe_amd.build`); "
+                           "the stage has no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in PROTOTYPES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SdbeError(RuntimeError):
+    def __init__(self, code, where, hip=0):
+        lib = load()
+        msg = lib.sdbe_error_string(code).decode()
+        super().__init__("%s failed: %s (code %d, hip error %d)" % (where, msg, code, hip))
+        self.code = code
+        self.hip = hip
+
+
+def check(code, where, handle=None):
+    if code != SDBE_OK:
+        hip = load().sdbe_last_hip_error(handle) if handle else 0
+        raise SdbeError(code, where, hip)
+    return code

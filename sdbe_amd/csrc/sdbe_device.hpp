@@ -1,0 +1,313 @@
+// Device helpers for the gfx950 SWARM data back-end stage: complex arithmetic,
+// twiddle tables, small DFTs (3, 4, 8, 13, 16, 39) in registers, and a
+// 16384-point Stockham FFT held in registers of a 1024-thread workgroup with
+// LDS exchanges between radix passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdbe {
+
+constexpr int NCH = 16384;            // complex channels per snapshot (= N/2 of the 32768 C2R)
+constexpr int NBLK = 39;              // snapshots per block
+constexpr int MP = NBLK * NCH;        // 638,976: complex length of the packed 1,277,952 R2C
+constexpr int P20 = 1 << 20;          // complex length of the packed 2,097,152 C2R
+constexpr int KHI = 64;               // P20 = KHI * NCH
+constexpr int FFT_THREADS = 1024;     // threads of the 16384-point FFT workgroups
+constexpr int LDS_PAD_N = NCH + NCH / 16;   // padded LDS row (one float per 16)
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by S*i (S = +1 or -1)
+template <int S> __device__ __forceinline__ float2 mul_si(float2 a) {
+    return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+// multiply by the constant (c + S*i*s)
+template <int S> __device__ __forceinline__ float2 cmulc(float2 a, float c, float s) {
+    const float ss = S > 0 ? s : -s;
+    return make_float2(a.x * c - a.y * ss, a.x * ss + a.y * c);
+}
+
+// Two-level table of exp(+2*pi*i*idx/N): value = hi[idx >> lo_bits] * lo[idx & mask].
+// Both levels are computed in FP64 on the host and rounded to FP32.
+struct Tw2 {
+    const float2 *hi;
+    const float2 *lo;
+    int lo_bits;
+};
+template <int S> __device__ __forceinline__ float2 tw(const Tw2 &t, uint32_t idx) {
+    float2 w = cmul(t.hi[idx >> t.lo_bits], t.lo[idx & ((1u << t.lo_bits) - 1u)]);
+    return S > 0 ? w : cconj(w);
+}
+
+// Single-level table of exp(+2*pi*i*m/16384), m = 0..16383 (FP64 -> FP32).
+template <int S> __device__ __forceinline__ float2 tw1(const float2 *T, uint32_t idx) {
+    const float2 w = T[idx];
+    return S > 0 ? w : cconj(w);
+}
+
+struct Tables {
+    const float2 *t16k;   // N = 16384, single level (Stockham twiddles)
+    Tw2 n32k;    // N = 32768        (C2R pre-twiddle of the 32768-point stage)
+    Tw2 nmp;     // N = 638,976      (inter-pass twiddle of the 1,277,952 R2C)
+    Tw2 nm;      // N = 1,277,952    (R2C post-processing)
+    Tw2 nl;      // N = 2,097,152    (C2R pre-processing of the final stage)
+    Tw2 np;      // N = 1,048,576    (inter-pass twiddle of the final stage)
+};
+
+// ---------------------------------------------------------------- small DFTs
+// Sign S: -1 forward (exp(-i...)), +1 inverse.  Inputs/outputs in natural order.
+template <int S> __device__ __forceinline__ void dft2(float2 &a, float2 &b) {
+    float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <int S> __device__ __forceinline__ void dft4(float2 &a, float2 &b, float2 &c, float2 &d) {
+    float2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = mul_si<S>(csub(b, d));
+    a = cadd(t0, t2);
+    c = csub(t0, t2);
+    b = cadd(t1, t3);
+    d = csub(t1, t3);
+}
+
+template <int S> __device__ __forceinline__ void dft3(float2 &a, float2 &b, float2 &c) {
+    const float s3 = 8.660254038e-01f;
+    float2 sbc = cadd(b, c), dbc = csub(b, c);
+    float2 m = make_float2(a.x - 0.5f * sbc.x, a.y - 0.5f * sbc.y);
+    float2 r = mul_si<S>(cscale(dbc, s3));
+    a = cadd(a, sbc);
+    b = cadd(m, r);
+    c = csub(m, r);
+}
+
+// DFT-8: n = 4*n1 + n2, k = k1 + 2*k2
+template <int S> __device__ __forceinline__ void dft8(float2 *v) {
+    const float h = 7.071067812e-01f;
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) dft2<S>(v[n2], v[4 + n2]);
+    v[5] = cmulc<S>(v[5], h, h);
+    v[6] = mul_si<S>(v[6]);
+    v[7] = cmulc<S>(v[7], -h, h);
+    dft4<S>(v[0], v[1], v[2], v[3]);
+    dft4<S>(v[4], v[5], v[6], v[7]);
+    float2 t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = v[4 * (k & 1) + (k >> 1)];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = t[k];
+}
+
+// DFT-16: n = 4*n1 + n2, k = k1 + 4*k2
+template <int S> __device__ __forceinline__ void dft16(float2 *v) {
+    const float c1 = 9.238795325e-01f, s1 = 3.826834324e-01f, h = 7.071067812e-01f;
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) dft4<S>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // twiddle W16^(n2*k1) at position 4*k1 + n2
+    v[5] = cmulc<S>(v[5], c1, s1);      // 1
+    v[6] = cmulc<S>(v[6], h, h);        // 2
+    v[7] = cmulc<S>(v[7], s1, c1);      // 3
+    v[9] = cmulc<S>(v[9], h, h);        // 2
+    v[10] = mul_si<S>(v[10]);           // 4
+    v[11] = cmulc<S>(v[11], -h, h);     // 6
+    v[13] = cmulc<S>(v[13], s1, c1);    // 3
+    v[14] = cmulc<S>(v[14], -h, h);     // 6
+    v[15] = cmulc<S>(v[15], -c1, -s1);  // 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4<S>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    float2 t[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t[k] = v[4 * (k & 3) + (k >> 2)];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = t[k];
+}
+
+// cos / sin of 2*pi*m/13 for m = 0..12
+__device__ __forceinline__ float c13(int m) {
+    const float c[7] = {1.0f, 8.854560257e-01f, 5.680647467e-01f, 1.205366803e-01f,
+                        -3.546048870e-01f, -7.485107482e-01f, -9.709418174e-01f};
+    return m <= 6 ? c[m] : c[13 - m];
+}
+__device__ __forceinline__ float s13(int m) {
+    const float s[7] = {0.0f, 4.647231720e-01f, 8.229838659e-01f, 9.927088741e-01f,
+                        9.350162427e-01f, 6.631226582e-01f, 2.393156643e-01f};
+    return m <= 6 ? s[m] : -s[13 - m];
+}
+
+// DFT-13 by conjugate pairing: X[d] = x0 + A_d + S*i*B_d, X[13-d] = x0 + A_d - S*i*B_d
+template <int S> __device__ __forceinline__ void dft13(float2 *x) {
+    float2 sp[7], dm[7];
+    float2 tot = x[0];
+#pragma unroll
+    for (int j = 1; j <= 6; ++j) {
+        sp[j] = cadd(x[j], x[13 - j]);
+        dm[j] = csub(x[j], x[13 - j]);
+        tot = cadd(tot, sp[j]);
+    }
+    float2 x0 = x[0];
+    x[0] = tot;
+#pragma unroll
+    for (int d = 1; d <= 6; ++d) {
+        float2 A = x0, B = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 1; j <= 6; ++j) {
+            const int m = (j * d) % 13;
+            A.x += sp[j].x * c13(m);
+            A.y += sp[j].y * c13(m);
+            B.x += dm[j].x * s13(m);
+            B.y += dm[j].y * s13(m);
+        }
+        float2 iB = mul_si<S>(B);
+        x[d] = cadd(A, iB);
+        x[13 - d] = csub(A, iB);
+    }
+}
+
+// exp(S*2*pi*i*b*c/39) for c = 1, 2 and b = 0..12 (cos, |sin|) -- sign applied by S
+__device__ __forceinline__ float2 w39(int b, int c) {
+    const float2 t1[13] = {{1.000000000e+00f, 0.0f}, {9.870502626e-01f, 1.604112809e-01f},
+                           {9.485364419e-01f, 3.166679938e-01f}, {8.854560257e-01f, 4.647231720e-01f},
+                           {7.994427634e-01f, 6.007422642e-01f}, {6.927243535e-01f, 7.212024473e-01f},
+                           {5.680647467e-01f, 8.229838659e-01f}, {4.286925614e-01f, 9.034504346e-01f},
+                           {2.782174639e-01f, 9.605181116e-01f}, {1.205366803e-01f, 9.927088741e-01f},
+                           {-4.026594011e-02f, 9.991889982e-01f}, {-2.000256938e-01f, 9.797906520e-01f},
+                           {-3.546048870e-01f, 9.350162427e-01f}};
+    const float2 t2[13] = {{1.000000000e+00f, 0.0f}, {9.485364419e-01f, 3.166679938e-01f},
+                           {7.994427634e-01f, 6.007422642e-01f}, {5.680647467e-01f, 8.229838659e-01f},
+                           {2.782174639e-01f, 9.605181116e-01f}, {-4.026594011e-02f, 9.991889982e-01f},
+                           {-3.546048870e-01f, 9.350162427e-01f}, {-6.324453756e-01f, 7.746049618e-01f},
+                           {-8.451900855e-01f, 5.344658261e-01f}, {-9.709418174e-01f, 2.393156643e-01f},
+                           {-9.967573081e-01f, -8.046656872e-02f}, {-9.199794437e-01f, -3.919666099e-01f},
+                           {-7.485107482e-01f, -6.631226582e-01f}};
+    return c == 1 ? t1[b] : t2[b];
+}
+
+// DFT-39 = 3 x 13: s = 13a + b, k = c + 3d.  In place; output in natural order.
+template <int S> __device__ __forceinline__ void dft39(float2 *x) {
+#pragma unroll
+    for (int b = 0; b < 13; ++b) dft3<S>(x[b], x[13 + b], x[26 + b]);
+#pragma unroll
+    for (int c = 1; c < 3; ++c) {
+#pragma unroll
+        for (int b = 1; b < 13; ++b) {
+            float2 w = w39(b, c);
+            x[13 * c + b] = cmulc<S>(x[13 * c + b], w.x, w.y);
+        }
+    }
+    dft13<S>(x);
+    dft13<S>(x + 13);
+    dft13<S>(x + 26);
+    float2 t[39];
+#pragma unroll
+    for (int k = 0; k < 39; ++k) t[k] = x[13 * (k % 3) + k / 3];
+#pragma unroll
+    for (int k = 0; k < 39; ++k) x[k] = t[k];
+}
+
+// -------------------------------------------------- 16384-point Stockham FFT
+// 1024 threads; thread t holds elements t + 1024*q (q = 0..15) on entry and on
+// exit (natural order both ways).  Passes: radix 16 (Ns = 1, 16, 256) then
+// radix 4 (Ns = 4096).  LDS holds real and imaginary parts in two padded
+// float arrays of LDS_PAD_N entries each.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+
+// v[r] *= w^r for r = 1..15 with w = T[b]; only w, w^2, w^4, w^8 come from the
+// table, the other powers are products of at most four table values.
+template <int S> __device__ __forceinline__ void twiddle16(float2 *v, const float2 *T, uint32_t b) {
+    const float2 w1 = tw1<S>(T, b), w2 = tw1<S>(T, 2u * b), w4 = tw1<S>(T, 4u * b), w8 = tw1<S>(T, 8u * b);
+    const float2 w3 = cmul(w1, w2), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+    v[1] = cmul(v[1], w1);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], w5);
+    v[6] = cmul(v[6], w6);
+    v[7] = cmul(v[7], w7);
+    v[8] = cmul(v[8], w8);
+    v[9] = cmul(v[9], cmul(w8, w1));
+    v[10] = cmul(v[10], cmul(w8, w2));
+    v[11] = cmul(v[11], cmul(w8, w3));
+    v[12] = cmul(v[12], cmul(w8, w4));
+    v[13] = cmul(v[13], cmul(w8, w5));
+    v[14] = cmul(v[14], cmul(w8, w6));
+    v[15] = cmul(v[15], cmul(w8, w7));
+}
+
+template <int S>
+__device__ __forceinline__ void fft16k(float2 *v, float *lre, float *lim, const float2 *t16) {
+    const int t = threadIdx.x;
+    // pass 1: radix 16, Ns = 1 (no twiddle)
+    dft16<S>(v);
+    __syncthreads();   // previous users of the LDS arrays are done
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        int o = lpad(16 * t + r);
+        lre[o] = v[r].x;
+        lim[o] = v[r].y;
+    }
+    __syncthreads();
+    // pass 2: radix 16, Ns = 16
+    {
+        const int k = t & 15;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            int i = lpad(t + 1024 * r);
+            v[r] = make_float2(lre[i], lim[i]);
+        }
+        twiddle16<S>(v, t16, (uint32_t)(64 * k));
+        dft16<S>(v);
+        __syncthreads();
+        const int base = (t >> 4) * 256 + k;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            int o = lpad(base + 16 * r);
+            lre[o] = v[r].x;
+            lim[o] = v[r].y;
+        }
+        __syncthreads();
+    }
+    // pass 3: radix 16, Ns = 256
+    {
+        const int k = t & 255;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            int i = lpad(t + 1024 * r);
+            v[r] = make_float2(lre[i], lim[i]);
+        }
+        twiddle16<S>(v, t16, (uint32_t)(4 * k));
+        dft16<S>(v);
+        __syncthreads();
+        const int base = (t >> 8) * 4096 + k;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            int o = lpad(base + 256 * r);
+            lre[o] = v[r].x;
+            lim[o] = v[r].y;
+        }
+        __syncthreads();
+    }
+    // pass 4: radix 4, Ns = 4096; butterfly j = t + 1024*b reads/writes j + 4096*r,
+    // i.e. element t + 1024*(b + 4r) -> register v[b + 4r]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        int i = lpad(t + 1024 * q);
+        v[q] = make_float2(lre[i], lim[i]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t j = (uint32_t)(t + 1024 * b);
+        const float2 w1 = tw1<S>(t16, j), w2 = tw1<S>(t16, 2u * j);
+        v[b + 4] = cmul(v[b + 4], w1);
+        v[b + 8] = cmul(v[b + 8], w2);
+        v[b + 12] = cmul(v[b + 12], cmul(w1, w2));
+        dft4<S>(v[b], v[b + 4], v[b + 8], v[b + 12]);
+    }
+}
+
+}  // namespace sdbe

@@ -1,0 +1,160 @@
+"""Python host side of the stage: a thin owner of one C-ABI handle.
+
+Device memory comes from torch (plumbing only); every computation runs in
+libsdbe.so.  ``SdbeStage`` mirrors the per-slot work of the reference's
+``vdif_inout_gpu_thread`` run loop (software/aphids/vdif_inout_gpu_thread.cu:
+874-1023): one call per group of 39 B-frames, output either as payloads
+(vdif_out_data_group_t layout) or as complete VDIF frames.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _stream(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def station_code(s):
+    return ((ord(s[0]) & 0xFF) << 8) | (ord(s[1]) & 0xFF)
+
+
+class SdbeStage:
+    def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=32, device=0,
+                 station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0)):
+        self.lib = L.load()
+        cfg = L.Config()
+        L.check(self.lib.sdbe_default_config(C.byref(cfg)), "sdbe_default_config")
+        cfg.thresh_policy = policy
+        cfg.out_mode = out_mode
+        cfg.chunk_blockpols = chunk
+        cfg.max_spectra_blockpols = max_spectra
+        cfg.station_id = station_code(station)
+        cfg.bdc_sideband = bdc
+        cfg.rx_sideband = rx
+        cfg.fixed_threshold = fixed[0]
+        cfg.fixed_offset = fixed[1]
+        self.cfg = cfg
+        self.device = device
+        self.out_mode = out_mode
+        self.frame_bytes = L.OUT_PAYLOAD_BYTES + (L.OUT_HEADER_BYTES if out_mode == L.OUT_FRAMES else 0)
+        h = C.c_void_p()
+        L.check(self.lib.sdbe_create(C.byref(cfg), device, C.byref(h)), "sdbe_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sdbe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- state
+    def set_state(self, secs=0, df=0, ref_epoch=0, psn=1):
+        st = L.StreamState(secs=secs, df=df, ref_epoch=ref_epoch, reserved=0, psn=psn)
+        L.check(self.lib.sdbe_set_stream_state(self.h, C.byref(st)), "sdbe_set_stream_state", self.h)
+
+    def get_state(self):
+        st = L.StreamState()
+        L.check(self.lib.sdbe_get_stream_state(self.h, C.byref(st)), "sdbe_get_stream_state", self.h)
+        return dict(secs=st.secs, df=st.df, ref_epoch=st.ref_epoch, psn=st.psn)
+
+    def stats(self):
+        s = L.Stats()
+        L.check(self.lib.sdbe_get_stats(self.h, C.byref(s)), "sdbe_get_stats", self.h)
+        return {k: getattr(s, k) for k, _ in L.Stats._fields_ if k != "reserved"}
+
+    def profile(self, enable=True):
+        L.check(self.lib.sdbe_profile_enable(self.h, 1 if enable else 0), "sdbe_profile_enable", self.h)
+
+    def profile_read(self):
+        """{kernel name: (launches, total ms, units)} since the last read."""
+        buf = (L.KernelTime * 32)()
+        n = C.c_int32()
+        L.check(self.lib.sdbe_profile_read(self.h, buf, 32, C.byref(n)), "sdbe_profile_read", self.h)
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms, buf[i].units)
+                for i in range(min(n.value, 32))}
+
+    # -------------------------------------------------------------- buffers
+    def group_out(self):
+        """Output buffer of one group, [2 pols][8192 frames][frame bytes] uint8."""
+        return torch.empty((2, L.OUT_FRAMES_PER_GROUP, self.frame_bytes), dtype=torch.uint8,
+                           device="cuda:%d" % self.device)
+
+    @staticmethod
+    def to_device_packets(pkts_u32, device=0):
+        t = torch.from_numpy(np.ascontiguousarray(pkts_u32).view(np.uint8).reshape(-1))
+        return t.to("cuda:%d" % device)
+
+    # ----------------------------------------------------------------- calls
+    def process_group(self, d_pkts, n_pkts, b_zero, out=None, bitmap=None, stream=None):
+        if out is None:
+            out = self.group_out()
+        assert out.is_contiguous() and out.numel() >= 2 * L.OUT_FRAMES_PER_GROUP * self.frame_bytes
+        assert d_pkts.numel() >= n_pkts * L.PKT_BYTES
+        stride = out.numel() // 2
+        rc = self.lib.sdbe_process_group(self.h, _ptr(d_pkts), n_pkts, b_zero, _ptr(out), stride,
+                                         _ptr(bitmap), _stream(stream))
+        L.check(rc, "sdbe_process_group", self.h)
+        return out
+
+    def unpack_int8(self, d_pkts, n_pkts, b_zero, stream=None):
+        dev = "cuda:%d" % self.device
+        b0 = torch.empty((39, 128, L.CHANNELS, 2), dtype=torch.int8, device=dev)
+        b1 = torch.empty_like(b0)
+        bm = torch.empty((39, 256), dtype=torch.uint8, device=dev)
+        assert d_pkts.numel() >= n_pkts * L.PKT_BYTES
+        rc = self.lib.sdbe_unpack_group_int8(self.h, _ptr(d_pkts), n_pkts, b_zero, _ptr(b0), _ptr(b1),
+                                             _ptr(bm), _stream(stream))
+        L.check(rc, "sdbe_unpack_group_int8", self.h)
+        return b0, b1, bm
+
+    def process_spectra(self, d_beng, out=None, stream=None):
+        """d_beng: int8 [n_bp][39][16384][2 = (im, re)] on the device."""
+        n_bp = d_beng.shape[0]
+        assert d_beng.dtype == torch.int8 and d_beng.is_contiguous()
+        assert tuple(d_beng.shape[1:]) == (39, L.CHANNELS, 2)
+        if out is None:
+            out = torch.empty((n_bp, L.OUT_FRAMES_PER_BLOCK, self.frame_bytes), dtype=torch.uint8,
+                              device=d_beng.device)
+        assert out.numel() >= n_bp * L.OUT_FRAMES_PER_BLOCK * self.frame_bytes
+        rc = self.lib.sdbe_process_spectra(self.h, _ptr(d_beng), n_bp, _ptr(out), _stream(stream))
+        L.check(rc, "sdbe_process_spectra", self.h)
+        return out
+
+    def resample_f32(self, d_beng, stream=None):
+        n_bp = d_beng.shape[0]
+        assert d_beng.dtype == torch.int8 and d_beng.is_contiguous()
+        assert tuple(d_beng.shape[1:]) == (39, L.CHANNELS, 2)
+        x = torch.empty((n_bp, L.SAMPLES_PER_BLOCK), dtype=torch.float32, device=d_beng.device)
+        st = torch.empty((n_bp, 2), dtype=torch.float64, device=d_beng.device)
+        rc = self.lib.sdbe_resample_spectra_f32(self.h, _ptr(d_beng), n_bp, _ptr(x), _ptr(st),
+                                                _stream(stream))
+        L.check(rc, "sdbe_resample_spectra_f32", self.h)
+        return x, st
+
+
+def state_from_template(template_words, n_words_per_chan=L.OUT_FRAMES_PER_GROUP * 2048):
+    lib = L.load()
+    w = (C.c_uint32 * 8)(*[int(x) for x in template_words[:8]])
+    st = L.StreamState()
+    sk = C.c_int32()
+    sf = C.c_int32()
+    L.check(lib.sdbe_stream_state_from_template(C.cast(w, C.c_void_p), n_words_per_chan, C.byref(st),
+                                                C.byref(sk), C.byref(sf)), "sdbe_stream_state_from_template")
+    return dict(secs=st.secs, df=st.df, ref_epoch=st.ref_epoch, psn=st.psn, skip_int32=sk.value,
+                skip_frames=sf.value)

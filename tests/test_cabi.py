@@ -1,0 +1,79 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every symbol
+include/sdbe_stage.h declares, and the ctypes structures match the C layout.
+No compute calls (no GPU here)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "sdbe_stage.h")
+
+
+def _declared_functions():
+    src = open(HDR).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(sdbe_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from sdbe_amd import _lib
+    lib = _lib.load()
+    declared = _declared_functions()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(lib, name), name
+    bound = {p[0] for p in _lib.PROTOTYPES}
+    assert set(declared) == bound
+
+
+def test_struct_layouts_match_c(tmp_path):
+    """Compile a probe against the header with gcc and compare sizeof/offsetof
+    with the ctypes mirrors."""
+    from sdbe_amd import _lib as L
+    probe = tmp_path / "probe.c"
+    probe.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "sdbe_stage.h"
+#define P(T, F) printf(#T "." #F " %zu\\n", offsetof(T, F));
+int main(void) {
+  printf("sdbe_config_t %zu\\n", sizeof(sdbe_config_t));
+  printf("sdbe_stream_state_t %zu\\n", sizeof(sdbe_stream_state_t));
+  printf("sdbe_stats_t %zu\\n", sizeof(sdbe_stats_t));
+  printf("sdbe_kernel_time_t %zu\\n", sizeof(sdbe_kernel_time_t));
+  P(sdbe_config_t, station_id) P(sdbe_config_t, rx_sideband) P(sdbe_config_t, reserved)
+  P(sdbe_stream_state_t, psn) P(sdbe_stats_t, latched_threshold) P(sdbe_kernel_time_t, total_ms)
+  return 0;
+}
+""")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(probe), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.split("\n") if l)
+    assert int(out["sdbe_config_t"]) == C.sizeof(L.Config)
+    assert int(out["sdbe_stream_state_t"]) == C.sizeof(L.StreamState)
+    assert int(out["sdbe_stats_t"]) == C.sizeof(L.Stats)
+    assert int(out["sdbe_kernel_time_t"]) == C.sizeof(L.KernelTime)
+    assert int(out["sdbe_config_t.station_id"]) == L.Config.station_id.offset
+    assert int(out["sdbe_config_t.rx_sideband"]) == L.Config.rx_sideband.offset
+    assert int(out["sdbe_config_t.reserved"]) == L.Config.reserved.offset
+    assert int(out["sdbe_stream_state_t.psn"]) == L.StreamState.psn.offset
+    assert int(out["sdbe_stats_t.latched_threshold"]) == L.Stats.latched_threshold.offset
+    assert int(out["sdbe_kernel_time_t.total_ms"]) == L.KernelTime.total_ms.offset
+
+
+def test_host_only_entry_points_without_gpu():
+    """Calls that touch no device: config defaults, error strings, template
+    state derivation, argument validation."""
+    from sdbe_amd import _lib as L
+    lib = L.load()
+    cfg = L.Config()
+    assert lib.sdbe_default_config(C.byref(cfg)) == 0
+    assert cfg.thresh_policy == L.THRESH_REFERENCE and cfg.station_id == 0x5858
+    assert lib.sdbe_default_config(None) == L.SDBE_ERR_ARG
+    assert lib.sdbe_error_string(L.SDBE_ERR_CAPACITY).decode() == "request exceeds handle capacity"
+    assert lib.sdbe_process_group(None, None, 0, 0, None, 0, None, None) == L.SDBE_ERR_ARG
+    assert lib.sdbe_get_stats(None, None) == L.SDBE_ERR_ARG

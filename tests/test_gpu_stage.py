@@ -1,0 +1,270 @@
+"""GPU parity tests: the HIP stage through its C-ABI against the FP64 oracle.
+
+Bars:
+* unpack / reorder / zero fill / fid bitmap / counters: bit-exact;
+* resampled FP32 samples: RMS error <= 2e-6 sigma and max error <= 5e-5 sigma
+  of the FP64 oracle (the FP32 pipeline's rounding; north-star bound 1e-3 sigma);
+* 2-bit codes: identical except samples within 1e-5 sigma of a decision
+  boundary (-th, 0, +th after the offset), which are counted;
+* VDIF headers: bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sdbe_oracle as O
+from sdbe_amd import synth as S
+
+pytestmark = pytest.mark.gpu
+
+BAND_SIGMA = 1e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _stage(**kw):
+    from sdbe_amd.stage import SdbeStage
+    return SdbeStage(**kw)
+
+
+def _spectra_codes(seed, n_bp, **kw):
+    rng = S.rng_for(seed)
+    codes = S.gaussian_codes(rng, 39 * n_bp, **kw)
+    return S.codes_to_beng_int8(codes).reshape(n_bp, 39, 16384, 2)
+
+
+def _code_mismatch_report(x64, codes_gpu, th, off):
+    """Codes the oracle assigns to FP64 samples vs the GPU codes; returns
+    (n_mismatch, n_outside_band, max distance of a mismatch to a boundary / sigma)."""
+    c64 = O.quantize2bit(x64, th, off)
+    bad = np.nonzero(c64 != codes_gpu)[0]
+    if bad.size == 0:
+        return 0, 0, 0.0
+    v = x64[bad] - off
+    dist = np.min(np.abs(np.stack([v - th, v, v + th])), axis=0) / th
+    return int(bad.size), int((dist > BAND_SIGMA).sum()), float(dist.max())
+
+
+# --------------------------------------------------------------------- unpack
+def test_unpack_group_bit_exact(torch_cuda):
+    _say("[unpack] generating one shuffled group with drops, duplicates, invalid and out-of-range packets")
+    rng = S.rng_for(11)
+    c0 = S.gaussian_codes(rng, 39 * 128)
+    c1 = S.gaussian_codes(rng, 39 * 128)
+    b0 = 5000
+    pk = S.packetize(c0, c1, b0)
+    pk, _ = S.shuffle_and_drop(pk, rng, drop_frac=0.001)
+    # duplicates with different payload placed after the original: first wins
+    dup = pk[:7].copy()
+    dup[:, 8:] ^= 0x55555555
+    inval = pk[7:9].copy()
+    inval[:, 0] |= np.uint32(1 << 31)
+    inval[:, 8:] = 0
+    far = pk[9:11].copy()
+    far[:, 4] += np.uint32(1)          # B-count + 256: outside the 39-frame window
+    pk = np.ascontiguousarray(np.concatenate([pk, dup, inval, far]))
+    _say("[unpack] oracle")
+    ref = O.vdif_to_beng(pk, b0)
+    st = _stage()
+    d = st.to_device_packets(pk)
+    _say("[unpack] gpu n_pkts=%d" % pk.shape[0])
+    g0, g1, bm = st.unpack_int8(d, pk.shape[0], b0)
+    stats = st.stats()
+    g0 = g0.cpu().numpy()
+    g1 = g1.cpu().numpy()
+    assert np.array_equal(g0, ref.beng[0])
+    assert np.array_equal(g1, ref.beng[1])
+    assert np.array_equal(bm.cpu().numpy(), ref.bitmap)
+    assert stats["n_invalid"] == ref.n_invalid == 2
+    assert stats["n_out_of_range"] == ref.n_out_of_range
+    assert stats["n_duplicate"] == ref.n_duplicate == 7
+    assert stats["n_unique"] == ref.n_unique
+    assert stats["n_missing"] == ref.n_missing == round(0.001 * 79872)
+    st.close()
+
+
+def test_unpack_wraparound_bcount(torch_cuda):
+    """B-counts straddling 2^32 of the 24+8-bit field the kernel reads, and a
+    b_zero above 2^32 (only the low 32 bits enter, vdif_inout_gpu_thread.cu:437)."""
+    rng = S.rng_for(12)
+    c0 = S.gaussian_codes(rng, 39 * 128, tones=())
+    c1 = S.gaussian_codes(rng, 39 * 128, tones=())
+    b0 = (1 << 32) - 20 + (7 << 32)
+    pk = S.packetize(c0, c1, b0)
+    pk, _ = S.shuffle_and_drop(pk, rng, drop_frac=0.0)
+    ref = O.vdif_to_beng(pk, b0)
+    assert ref.n_unique == 79872
+    st = _stage()
+    d = st.to_device_packets(pk)
+    _say("[unpack-wrap] gpu")
+    g0, g1, bm = st.unpack_int8(d, pk.shape[0], b0)
+    assert np.array_equal(g0.cpu().numpy(), ref.beng[0])
+    assert np.array_equal(g1.cpu().numpy(), ref.beng[1])
+    st.close()
+
+
+# ------------------------------------------------------------------ resampler
+def test_resample_f32_vs_fp64(torch_cuda):
+    torch = torch_cuda
+    n_bp = 3
+    beng = _spectra_codes(21, n_bp)
+    st = _stage(policy=1, max_spectra=n_bp)
+    _say("[resample] gpu n_bp=%d" % n_bp)
+    x, stt = st.resample_f32(torch.from_numpy(beng).cuda())
+    x = x.cpu().numpy().astype(np.float64)
+    stt = stt.cpu().numpy()
+    for b in range(n_bp):
+        x64 = O.resample_block(O.inflate(beng[b]))
+        mean, sig = O.block_stats(x64)
+        err = x[b] - x64
+        rms = np.sqrt(np.mean(err ** 2)) / sig
+        mx = np.abs(err).max() / sig
+        _say("[resample] bp %d rms %.3g sigma, max %.3g sigma" % (b, rms, mx))
+        assert rms <= 2e-6
+        assert mx <= 5e-5
+        assert abs(stt[b, 0] - mean) <= 1e-6 * sig
+        assert abs(stt[b, 1] - sig) <= 1e-6 * sig
+    st.close()
+
+
+def test_resample_pure_tone_is_analytic(torch_cuda):
+    """Channel k = 1 + 0j in every snapshot is a cosine continuous across the
+    block, i.e. bin 39 k of the 1,277,952-point DFT, so the resampled output is
+    the analytic (2/32768) cos(2 pi 39 k t / 2^21): pins the composition and
+    the numpy normalisation of the whole GPU resampler."""
+    torch = torch_cuda
+    k = 100
+    spec = np.zeros((1, 39, 16384, 2), np.int8)
+    spec[0, :, k, 1] = 1
+    st = _stage(policy=1, max_spectra=1)
+    _say("[tone] gpu")
+    x, stt = st.resample_f32(torch.from_numpy(spec).cuda())
+    x = x.cpu().numpy()[0].astype(np.float64)
+    t = np.arange(O.L_C2R)
+    za = (2.0 / 32768) * np.cos(2 * np.pi * k * 39 * t / O.L_C2R)
+    err = np.abs(x - za).max() / np.abs(za).max()
+    _say("[tone] max rel err %.3g" % err)
+    assert err < 2e-5
+    assert abs(stt.cpu().numpy()[0, 1] - za.std(ddof=1)) < 1e-6 * za.std()
+    st.close()
+
+
+def test_process_spectra_codes_and_headers(torch_cuda):
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    n_bp = 2
+    beng = _spectra_codes(31, n_bp)
+    st = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_FRAMES, max_spectra=n_bp, station="Sm",
+                bdc=1, rx=0)
+    st.set_state(secs=12345, df=124990, ref_epoch=30, psn=77)
+    _say("[codes] gpu process_spectra n_bp=%d" % n_bp)
+    out = st.process_spectra(torch.from_numpy(beng).cuda()).cpu().numpy()
+    new_state = st.get_state()
+    hdr = out[:, :, :32].copy().view(np.uint32).reshape(n_bp * 64, 8)
+    pay = out[:, :, 32:].copy().view(np.uint32).reshape(n_bp, -1)
+    state = O.StreamState(secs=12345, df=124990, ref_epoch=30, psn=77,
+                          station=O.station_from_str("Sm"), bdc=1, rx=0)
+    ref_hdr, ref_next = O.stamp_headers(n_bp * 64, state)
+    assert np.array_equal(hdr, ref_hdr[0])
+    assert new_state == dict(secs=ref_next.secs, df=ref_next.df, ref_epoch=30, psn=ref_next.psn)
+    for b in range(n_bp):
+        x64 = O.resample_block(O.inflate(beng[b]))
+        mean, sig = O.block_stats(x64)
+        codes = O.unpack_codes(pay[b])
+        n_bad, n_out, dmax = _code_mismatch_report(x64, codes, sig, mean)
+        _say("[codes] bp %d: %d codes differ, %d outside %.0e sigma (max %.3g)" % (b, n_bad, n_out, BAND_SIGMA, dmax))
+        assert n_out == 0
+        assert n_bad <= 2e-6 * x64.size + 10
+        frac = np.bincount(codes, minlength=4) / codes.size
+        assert abs(frac[0] - 0.1587) < 0.01 and abs(frac[3] - 0.1587) < 0.01
+    st.close()
+
+
+# ------------------------------------------------------------------ end to end
+@pytest.fixture(scope="module")
+def one_group():
+    rng = S.rng_for(41)
+    c0 = S.gaussian_codes(rng, 39 * 128)
+    c1 = S.gaussian_codes(rng, 39 * 128)
+    b0 = 777
+    pk = S.packetize(c0, c1, b0)
+    pk_shuf, _ = S.shuffle_and_drop(pk, rng, drop_frac=0.001)
+    return pk, pk_shuf, b0
+
+
+def test_process_group_reference_policy(torch_cuda, one_group):
+    from sdbe_amd import _lib as L
+    pk, pk_shuf, b0 = one_group
+    st = _stage(policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD)
+    d = st.to_device_packets(pk_shuf)
+    _say("[group] gpu process_group (REFERENCE policy)")
+    out = st.process_group(d, pk_shuf.shape[0], b0).cpu().numpy()
+    stats = st.stats()
+    assert stats["n_missing"] == round(0.001 * 79872)
+    ref = O.vdif_to_beng(pk_shuf, b0)
+    blocks = [0, 1, 64, 127]
+    _say("[group] oracle blocks %s" % blocks)
+    latch = [None]
+    x00 = O.resample_block(O.block_spectra(ref.beng[0], 0))
+    mean0, sig0 = O.block_stats(x00)
+    assert abs(stats["latched_threshold"] - sig0) <= 2e-6 * sig0
+    assert abs(stats["latched_offset"] - mean0) <= 2e-6 * sig0
+    pay = out.view(np.uint32).reshape(2, 8192, 2048)
+    for blk in blocks:
+        for pol in (0, 1):
+            x64 = x00 if (blk, pol) == (0, 0) else O.resample_block(O.block_spectra(ref.beng[pol], blk))
+            codes = O.unpack_codes(pay[pol, blk * 64:(blk + 1) * 64])
+            n_bad, n_out, dmax = _code_mismatch_report(x64, codes, sig0, mean0)
+            _say("[group] blk %d pol %d: %d differ, %d outside band" % (blk, pol, n_bad, n_out))
+            assert n_out == 0
+    st.close()
+
+
+def test_process_group_order_invariant_and_deterministic(torch_cuda, one_group):
+    """Arrival order does not change the output (index-addressed reorder), and a
+    repeated call is bit-identical."""
+    from sdbe_amd import _lib as L
+    pk, _, b0 = one_group
+    rng = S.rng_for(43)
+    pa, _ = S.shuffle_and_drop(pk, rng, drop_frac=0.0)
+    pb, _ = S.shuffle_and_drop(pk, rng, drop_frac=0.0)
+    st = _stage(policy=L.THRESH_PER_BLOCK)
+    _say("[group-inv] gpu x3")
+    oa = st.process_group(st.to_device_packets(pa), pa.shape[0], b0).cpu().numpy()
+    ob = st.process_group(st.to_device_packets(pb), pb.shape[0], b0).cpu().numpy()
+    oc = st.process_group(st.to_device_packets(pk), pk.shape[0], b0).cpu().numpy()
+    assert np.array_equal(oa, ob)
+    assert np.array_equal(oa, oc)
+    st.close()
+
+
+def test_process_group_frames_headers(torch_cuda, one_group):
+    from sdbe_amd import _lib as L
+    pk, pk_shuf, b0 = one_group
+    st = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_FRAMES, station="Sm", bdc=1, rx=1)
+    st.set_state(secs=100, df=120000, ref_epoch=30, psn=1)
+    _say("[group-frames] gpu")
+    out = st.process_group(st.to_device_packets(pk_shuf), pk_shuf.shape[0], b0).cpu().numpy()
+    hdr = out[:, :, :32].copy().view(np.uint32).reshape(2, 8192, 8)
+    state = O.StreamState(secs=100, df=120000, ref_epoch=30, psn=1,
+                          station=O.station_from_str("Sm"), bdc=1, rx=1)
+    ref_hdr, _ = O.stamp_headers(8192, state)
+    assert np.array_equal(hdr, ref_hdr)
+    # payloads equal the PAYLOAD-mode output
+    st2 = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD)
+    o2 = st2.process_group(st2.to_device_packets(pk_shuf), pk_shuf.shape[0], b0).cpu().numpy()
+    assert np.array_equal(out[:, :, 32:], o2)
+    st.close()
+    st2.close()

@@ -224,11 +224,23 @@ k_stage_a(SrcDesc sd, int bp0, float2 *__restrict__ V, Tables tb) {
 
 // Stage B: row k1 of block-pol bpl: w = IFFT16384(V[k1]); w[n] *= e^{-2 pi i n k1/638976};
 // Zf[k1 + 39 k2] = FFT16384(w)[k2]   (packed forward DFT of the 1,277,952 samples)
+// Workgroups are dispatched round-robin over the 8 XCDs; with xcd_map the 39
+// rows of a block-pol all run on one XCD so that the stride-39 stores of its
+// rows meet in that XCD's L2 (requires the chunk to be a multiple of 8).
 __global__ void __launch_bounds__(1024)
-k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb) {
+k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int xcd_map) {
     __shared__ float lre[LDS_PAD_N];
     __shared__ float lim[LDS_PAD_N];
-    const int k1 = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
+    const int lin = blockIdx.x, t = threadIdx.x;
+    int k1, bpl;
+    if (xcd_map) {
+        const int slot = lin >> 3;
+        bpl = (lin & 7) + 8 * (slot / NBLK);
+        k1 = slot % NBLK;
+    } else {
+        bpl = lin / NBLK;
+        k1 = lin % NBLK;
+    }
     const float2 *src = V + ((size_t)bpl * NBLK + k1) * NCH;
     float2 v[16];
 #pragma unroll
@@ -248,13 +260,17 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb) {
 // (k = c + 16384 kh) of the packed 2^20-point C2R input, all 64 rows.
 //   Y[m]  = 1/2 (Zf[m] + conj Zf[M'-m]) - 1/2 i e^{-2 pi i m/M} (Zf[m] - conj Zf[M'-m]),  m <= M'
 //   Zc[k] = (C[k] + conj C[P-k]) + i e^{2 pi i k/L} (C[k] - conj C[P-k]),  C = Y zero-padded
-//   Bc[tx][c] = e^{2 pi i c tx/P} sum_kh Zc[c + 16384 kh] e^{2 pi i kh tx/64}
+//   Bc[tx][c] = sum_kh Zc[c + 16384 kh] e^{2 pi i kh tx/64}
+// (the inter-pass twiddle e^{2 pi i c tx/P} is applied by stage C2 on load)
 // and the FP64 Parseval sum  sum_m w_m |Y[m]|^2 (w_0 = 1 on Re Y[0], else 2).
+// Twiddles factor per column and per row: e^{-2 pi i (c + 16384 r)/M} =
+// e^{-2 pi i c/M} e^{-2 pi i r/78}, e^{2 pi i (c + 16384 kh)/L} = e^{2 pi i c/L} e^{2 pi i kh/128}.
 __global__ void __launch_bounds__(256)
 k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__restrict__ part, Tables tb) {
     __shared__ float2 sz[32][41];
     __shared__ float2 sy[32][41];
     __shared__ float2 sc[32][65];
+    __shared__ float2 colm[32], coll[32], rowm[40], rowl[64], w64[64];
     __shared__ double red[2][4];
     const int tile = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
     const float2 *zf = Zf + (size_t)bpl * MP;
@@ -265,6 +281,20 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
         c = col < 16 ? p : ((NCH - p) & (NCH - 1));
         dup = (col >= 16) && (p == 0 || p == NCH / 2);
     };
+    if (t < 32) {
+        int c;
+        bool valid, dup;
+        col_c(t, c, valid, dup);
+        colm[t] = tw<-1>(tb.nm, (uint32_t)c);
+        coll[t] = tw<+1>(tb.nl, (uint32_t)c);
+    } else if (t < 72) {
+        rowm[t - 32] = tw<-1>(tb.nm, (uint32_t)(NCH * (t - 32)));
+    } else if (t < 136) {
+        rowl[t - 72] = tw<+1>(tb.nl, (uint32_t)(NCH * (t - 72)));
+    } else if (t < 200) {
+        const int e = t - 136;   // W64^(r j) for r = e >> 3, j = e & 7
+        w64[e] = tw<+1>(tb.np, (uint32_t)((e >> 3) * (e & 7) * NCH));
+    }
     // load Zf rows 0..39
     for (int e = t; e < 40 * 32; e += 256) {
         const int r = e >> 5, col = e & 31;
@@ -293,7 +323,7 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
             const float2 zp = cconj(sz[pcol][pr]);
             const float2 E = cscale(cadd(zm, zp), 0.5f);
             const float2 D = cscale(csub(zm, zp), 0.5f);
-            Y = cadd(E, mul_si<-1>(cmul(tw<-1>(tb.nm, (uint32_t)m), D)));
+            Y = cadd(E, mul_si<-1>(cmul(cmul(colm[col], rowm[r]), D)));
             if (!dup) {
                 if (m == 0) {
                     acc += (double)Y.x * (double)Y.x;
@@ -309,15 +339,12 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
     // Zc rows 0..63
     for (int e = t; e < 64 * 32; e += 256) {
         const int kh = e >> 5, col = e & 31;
-        int c;
-        bool valid, dup;
-        col_c(col, c, valid, dup);
+        const int c = col < 16 ? tile * 16 + col : ((NCH - (tile * 16 + (col & 15))) & (NCH - 1));
         const int pcol = col ^ 16;
         const int pr = (c == 0) ? (64 - kh) : (63 - kh);
         const float2 ck = kh <= 39 ? sy[col][kh] : make_float2(0.f, 0.f);
         const float2 cp = pr <= 39 ? cconj(sy[pcol][pr]) : make_float2(0.f, 0.f);
-        const uint32_t k = (uint32_t)(c + NCH * kh);
-        sc[col][kh] = cadd(cadd(ck, cp), mul_si<+1>(cmul(tw<+1>(tb.nl, k), csub(ck, cp))));
+        sc[col][kh] = cadd(cadd(ck, cp), mul_si<+1>(cmul(cmul(coll[col], rowl[kh]), csub(ck, cp))));
     }
     __syncthreads();
     // 64-point inverse DFT over kh per column: 8 threads x radix 8, two passes
@@ -334,17 +361,11 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
 #pragma unroll
-        for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], tw<+1>(tb.np, (uint32_t)(r * j * NCH)));
+        for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], w64[8 * r + j]);
         dft8<+1>(v);
-        int c;
-        bool valid, dup;
-        col_c(col, c, valid, dup);
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int tx = j + 8 * r;
-            sc[col][tx] = cmul(v[r], tw<+1>(tb.np, (uint32_t)(c * tx)));
-        }
+        for (int r = 0; r < 8; ++r) sc[col][j + 8 * r] = v[r];
     }
     __syncthreads();
     // write Bc[tx][c]: half-waves of 16 consecutive (lower) / reversed (upper) columns
@@ -385,62 +406,62 @@ struct StatsDesc {
 };
 
 __global__ void __launch_bounds__(256)
-k_stats(const double *__restrict__ part, int nbpl, int bp0, StatsDesc sdsc, float *__restrict__ latch,
-        float2 *__restrict__ thr, double *__restrict__ stats_out) {
+k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, double *__restrict__ stats_out) {
     __shared__ double red[2][256];
-    __shared__ double mean_s[256], sig_s[256];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, b = blockIdx.x;
     const double L = 2097152.0, FA = 32768.0;
-    for (int b = 0; b < nbpl; ++b) {
-        double a = 0.0, d = 0.0;
-        for (int i = t; i < C1_TILES; i += 256) {
-            a += part[((size_t)b * C1_TILES + i) * 2];
-            d += part[((size_t)b * C1_TILES + i) * 2 + 1];
-        }
-        red[0][t] = a;
-        red[1][t] = d;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (t < s) {
-                red[0][t] += red[0][t + s];
-                red[1][t] += red[1][t + s];
-            }
-            __syncthreads();
-        }
-        if (t == 0) {
-            const double sumsq = red[0][0] / (FA * FA * L);
-            const double mean = red[1][0] / (FA * L);
-            double var = (sumsq - L * mean * mean) / (L - 1.0);
-            if (var < 0.0) var = 0.0;
-            mean_s[b] = mean;
-            sig_s[b] = sqrt(var);
-            if (stats_out) {
-                stats_out[(size_t)(bp0 + b) * 2] = mean;
-                stats_out[(size_t)(bp0 + b) * 2 + 1] = sig_s[b];
-            }
+    double a = 0.0, d = 0.0;
+    for (int i = t; i < C1_TILES; i += 256) {
+        a += part[((size_t)b * C1_TILES + i) * 2];
+        d += part[((size_t)b * C1_TILES + i) * 2 + 1];
+    }
+    red[0][t] = a;
+    red[1][t] = d;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            red[0][t] += red[0][t + s];
+            red[1][t] += red[1][t + s];
         }
         __syncthreads();
     }
     if (t == 0) {
-        for (int b = 0; b < nbpl; ++b) {
-            float th, off;
-            const int pol = sdsc.group_mode ? ((bp0 + b) & 1) : 0;
-            if (sdsc.policy == SDBE_THRESH_PER_BLOCK) {
-                th = (float)sig_s[b];
-                off = (float)mean_s[b];
-            } else if (sdsc.policy == SDBE_THRESH_REFERENCE) {
-                if (latch[0] == 0.0f && pol == 0) {
-                    latch[0] = (float)sig_s[b];
-                    latch[1] = (float)mean_s[b];
-                }
-                th = latch[0];
-                off = latch[1];
-            } else {
-                th = sdsc.fixed_th;
-                off = sdsc.fixed_off;
-            }
-            thr[b] = make_float2(th, off);
+        const double sumsq = red[0][0] / (FA * FA * L);
+        const double mean = red[1][0] / (FA * L);
+        double var = (sumsq - L * mean * mean) / (L - 1.0);
+        if (var < 0.0) var = 0.0;
+        const double sig = sqrt(var);
+        ms[b] = make_double2(mean, sig);
+        if (stats_out) {
+            stats_out[(size_t)(bp0 + b) * 2] = mean;
+            stats_out[(size_t)(bp0 + b) * 2 + 1] = sig;
         }
+    }
+}
+
+// Threshold policy over a chunk, in block order (one thread: the REFERENCE
+// latch is sequential by definition).
+__global__ void k_thresh(const double2 *__restrict__ ms, int nbpl, int bp0, StatsDesc sdsc,
+                         float *__restrict__ latch, float2 *__restrict__ thr) {
+    if (threadIdx.x != 0) return;
+    for (int b = 0; b < nbpl; ++b) {
+        float th, off;
+        const int pol = sdsc.group_mode ? ((bp0 + b) & 1) : 0;
+        if (sdsc.policy == SDBE_THRESH_PER_BLOCK) {
+            th = (float)ms[b].y;
+            off = (float)ms[b].x;
+        } else if (sdsc.policy == SDBE_THRESH_REFERENCE) {
+            if (latch[0] == 0.0f && pol == 0) {
+                latch[0] = (float)ms[b].y;
+                latch[1] = (float)ms[b].x;
+            }
+            th = latch[0];
+            off = latch[1];
+        } else {
+            th = sdsc.fixed_th;
+            off = sdsc.fixed_off;
+        }
+        thr[b] = make_float2(th, off);
     }
 }
 
@@ -454,7 +475,7 @@ struct OutDesc {
 };
 
 // Stage C2: workgroup = (tx, block-pol).
-// z[tx + 64 ty] = sum_c Bc[tx][c] e^{2 pi i c ty/16384} is the packed output:
+// z[tx + 64 ty] = sum_c Bc[tx][c] e^{2 pi i c tx/2^20} e^{2 pi i c ty/16384} is the packed output:
 // samples (2t, 2t+1) = (Re, Im) z[t], scaled by 2^-36 to numpy units.  Codes
 // per vdif_inout_gpu_thread.cu:577 (v = x - off; ((|v| >= th) | 2) ^ 3 (v < 0));
 // the 4-bit pair code (re | im << 2) of t goes to stage[bp][tx][ty].
@@ -470,6 +491,11 @@ k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = src[t + 1024 * q];
+    // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
+    // times a workgroup-uniform one
+    const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
     fft16k<+1>(v, lre, lim, tb.t16k);
     if (F32) {
         float2 *dst = reinterpret_cast<float2 *>(od.f32) + (size_t)bp * P20 + tx + KHI * t;
@@ -569,12 +595,12 @@ __global__ void __launch_bounds__(256) k_headers(OutDesc od, HdrDesc hd) {
 
 // ======================================================================= host
 enum KernelId {
-    KID_FILL, KID_MAP, KID_REORDER, KID_EXPAND, KID_A, KID_B, KID_C1, KID_STATS, KID_C2, KID_PACK,
-    KID_HEADERS, KID_N
+    KID_FILL, KID_MAP, KID_REORDER, KID_EXPAND, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
+    KID_PACK, KID_HEADERS, KID_N
 };
 static const char *const kKernelNames[KID_N] = {
     "k_fill_i32", "k_packet_map", "k_reorder", "k_expand_int8", "k_stage_a", "k_stage_b",
-    "k_stage_c1", "k_stats", "k_stage_c2", "k_pack", "k_headers"};
+    "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers"};
 
 struct ProfRec {
     int kid;
@@ -600,6 +626,7 @@ struct sdbe_handle {
     float2 *V = nullptr, *Zf = nullptr, *Bc = nullptr;
     double *part = nullptr;
     float2 *thr = nullptr;
+    double2 *ms = nullptr;
     uint8_t *stage = nullptr;
     // pinned host mirrors
     int *h_cnt = nullptr;
@@ -719,11 +746,13 @@ int run_resampler(sdbe_handle_t *h, const SrcDesc &sd, int n_bp, const OutDesc &
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         LAUNCH(h, KID_A, nb, s, k_stage_a, dim3(NCH / 256, nb), dim3(256), sd, bp0, h->V, h->tb);
-        LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK, nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb);
+        LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
+               (nb % 8) == 0 ? 1 : 0);
         LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(C1_TILES, nb), dim3(256), (const float2 *)h->Zf, h->Bc,
                h->part, h->tb);
-        LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(1), dim3(256), (const double *)h->part, nb, bp0, sdsc,
-               h->latch, h->thr, d_stats);
+        LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
+        LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(64), (const double2 *)h->ms, nb, bp0, sdsc, h->latch,
+               h->thr);
         if (od.f32) {
             LAUNCH(h, KID_C2, nb, s, k_stage_c2<true>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
                    (const float2 *)h->thr, bp0, od, h->stage, h->tb);
@@ -813,6 +842,7 @@ void sdbe_destroy(sdbe_handle_t *h) {
     (void)hipFree(h->part);
     (void)hipFree(h->thr);
     (void)hipFree(h->stage);
+    (void)hipFree(h->ms);
     (void)hipHostFree(h->h_cnt);
     (void)hipHostFree(h->h_latch);
     for (auto &r : h->recs) {
@@ -866,6 +896,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->part, C * C1_TILES * 2 * sizeof(double));
     ALLOC(h->thr, C * sizeof(float2));
     ALLOC(h->stage, C * KHI * NCH);
+    ALLOC(h->ms, C * sizeof(double2));
 #undef ALLOC
     if (hipHostMalloc((void **)&h->h_cnt, CNT_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&h->h_latch, 2 * sizeof(float), hipHostMallocDefault) != hipSuccess) {

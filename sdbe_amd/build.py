@@ -14,6 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsdbe.so")
+HASHPIPE_LIB = os.path.join(HERE, "libsdbe_hashpipe.so")
+INCLUDE = os.path.join(ROOT, "include")
+COMPAT = os.path.join(CSRC, "hashpipe_compat")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -46,8 +49,24 @@ def build_stage_lib():
     return LIB
 
 
+def build_hashpipe_plugin():
+    """vdif_inout_gpu_thread plugin + host bookkeeping, against the in-process
+    hashpipe API stand-in (production builds use the real hashpipe headers)."""
+    tmp = HASHPIPE_LIB + ".tmp"
+    srcs = [os.path.join(CSRC, "vdif_inout_gpu_thread.c"), os.path.join(CSRC, "aphids_host.c"),
+            os.path.join(COMPAT, "hashpipe_compat.c")]
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    # plain C against the HIP runtime API (the HIP headers need the platform macro)
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-I", INCLUDE, "-I", COMPAT, "-I", os.path.join(rocm, "include"), "-o", tmp, *srcs,
+          "-L", HERE, "-lsdbe", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"), "-lpthread", "-lm"])
+    os.replace(tmp, HASHPIPE_LIB)
+    return HASHPIPE_LIB
+
+
 def build_all(verbose=False):
-    built = [build_stage_lib()]
+    built = [build_stage_lib(), build_hashpipe_plugin()]
     if verbose:
         for b in built:
             print("built", os.path.relpath(b, ROOT))

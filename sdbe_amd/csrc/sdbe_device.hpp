@@ -53,7 +53,7 @@ template <int S> __device__ __forceinline__ float2 tw1(const float2 *T, uint32_t
 }
 
 struct Tables {
-    const float2 *t16k;   // N = 16384, single level (Stockham twiddles)
+    const float2 *fft;    // FftTw image (640 entries) for the 16384-point FFT
     Tw2 n32k;    // N = 32768        (C2R pre-twiddle of the 32768-point stage)
     Tw2 nmp;     // N = 638,976      (inter-pass twiddle of the 1,277,952 R2C)
     Tw2 nm;      // N = 1,277,952    (R2C post-processing)
@@ -210,17 +210,30 @@ template <int S> __device__ __forceinline__ void dft39(float2 *x) {
     for (int k = 0; k < 39; ++k) x[k] = t[k];
 }
 
-// -------------------------------------------------- 16384-point Stockham FFT
-// 1024 threads; thread t holds elements t + 1024*q (q = 0..15) on entry and on
-// exit (natural order both ways).  Passes: radix 16 (Ns = 1, 16, 256) then
-// radix 4 (Ns = 4096).  LDS holds real and imaginary parts in two padded
-// float arrays of LDS_PAD_N entries each.
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+// ------------------------------------------- LDS twiddles of the 16384-point FFT
+// exp(+2 pi i m / N) for the three twiddled passes, computed in FP64 on the
+// host (640 entries, this order) and copied to LDS once per workgroup:
+//   t256[m]            N = 256,   m < 256       (radix-16 pass, Ns = 16)
+//   h4k[m>>6] l4k[m&63]   N = 4096,  m < 4096   (radix-16 pass, Ns = 256)
+//   h16k[m>>7] l16k[m&127] N = 16384, m < 16384 (radix-4 pass, Ns = 4096)
+struct FftTw {
+    float2 t256[256];
+    float2 h4k[64];
+    float2 l4k[64];
+    float2 h16k[128];
+    float2 l16k[128];
+};
+constexpr int FFT_TW_N = 640;
 
-// v[r] *= w^r for r = 1..15 with w = T[b]; only w, w^2, w^4, w^8 come from the
-// table, the other powers are products of at most four table values.
-template <int S> __device__ __forceinline__ void twiddle16(float2 *v, const float2 *T, uint32_t b) {
-    const float2 w1 = tw1<S>(T, b), w2 = tw1<S>(T, 2u * b), w4 = tw1<S>(T, 4u * b), w8 = tw1<S>(T, 8u * b);
+__device__ __forceinline__ void fft_tw_load(FftTw *L, const float2 *g) {
+    float2 *d = reinterpret_cast<float2 *>(L);
+    for (int i = threadIdx.x; i < FFT_TW_N; i += blockDim.x) d[i] = g[i];
+}
+template <int S> __device__ __forceinline__ float2 ctw(float2 w) { return S > 0 ? w : cconj(w); }
+
+// v[r] *= w^r (r = 1..15) given w, w^2, w^4, w^8; the other powers are
+// products of at most four of them.
+__device__ __forceinline__ void twiddle16_pow(float2 *v, float2 w1, float2 w2, float2 w4, float2 w8) {
     const float2 w3 = cmul(w1, w2), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
     v[1] = cmul(v[1], w1);
     v[2] = cmul(v[2], w2);
@@ -239,36 +252,49 @@ template <int S> __device__ __forceinline__ void twiddle16(float2 *v, const floa
     v[15] = cmul(v[15], cmul(w8, w7));
 }
 
+// -------------------------------------------------- 16384-point Stockham FFT
+// 1024 threads; thread t holds elements t + 1024*q (q = 0..15) on entry and on
+// exit (natural order both ways).  Passes: radix 16 (Ns = 1, 16, 256) then
+// radix 4 (Ns = 4096).  LDS holds real and imaginary parts in two padded
+// float arrays of LDS_PAD_N entries each.
+// Padded LDS position of element i is i + (i >> 4).  Every access below is a
+// per-thread base plus a compile-time constant, so no index arithmetic runs
+// per element:
+//   reads  of t + 1024 r                    -> (t + (t >> 4)) + 1088 r
+//   pass-1 writes of 16 t + r               -> 17 t + r
+//   pass-2 writes of (t>>4) 256 + k + 16 r  -> (t>>4) 272 + k + 17 r      (k = t & 15)
+//   pass-3 writes of (t>>8) 4096 + k + 256 r -> (t>>8) 4352 + k + (k>>4) + 272 r  (k = t & 255)
 template <int S>
-__device__ __forceinline__ void fft16k(float2 *v, float *lre, float *lim, const float2 *t16) {
+__device__ __forceinline__ void fft16k(float2 *v, float *lre, float *lim, const FftTw *tw) {
     const int t = threadIdx.x;
+    const int rd = t + (t >> 4);
     // pass 1: radix 16, Ns = 1 (no twiddle)
     dft16<S>(v);
     __syncthreads();   // previous users of the LDS arrays are done
+    {
+        float *wr = lre + 17 * t, *wi = lim + 17 * t;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        int o = lpad(16 * t + r);
-        lre[o] = v[r].x;
-        lim[o] = v[r].y;
+        for (int r = 0; r < 16; ++r) {
+            wr[r] = v[r].x;
+            wi[r] = v[r].y;
+        }
     }
     __syncthreads();
     // pass 2: radix 16, Ns = 16
     {
         const int k = t & 15;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            int i = lpad(t + 1024 * r);
-            v[r] = make_float2(lre[i], lim[i]);
-        }
-        twiddle16<S>(v, t16, (uint32_t)(64 * k));
+        for (int r = 0; r < 16; ++r) v[r] = make_float2(lre[rd + 1088 * r], lim[rd + 1088 * r]);
+        twiddle16_pow(v, ctw<S>(tw->t256[k]), ctw<S>(tw->t256[2 * k]), ctw<S>(tw->t256[4 * k]),
+                      ctw<S>(tw->t256[8 * k]));
         dft16<S>(v);
         __syncthreads();
-        const int base = (t >> 4) * 256 + k;
+        const int base = (t >> 4) * 272 + k;
+        float *wr = lre + base, *wi = lim + base;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            int o = lpad(base + 16 * r);
-            lre[o] = v[r].x;
-            lim[o] = v[r].y;
+            wr[17 * r] = v[r].x;
+            wi[17 * r] = v[r].y;
         }
         __syncthreads();
     }
@@ -276,33 +302,33 @@ __device__ __forceinline__ void fft16k(float2 *v, float *lre, float *lim, const 
     {
         const int k = t & 255;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            int i = lpad(t + 1024 * r);
-            v[r] = make_float2(lre[i], lim[i]);
-        }
-        twiddle16<S>(v, t16, (uint32_t)(4 * k));
+        for (int r = 0; r < 16; ++r) v[r] = make_float2(lre[rd + 1088 * r], lim[rd + 1088 * r]);
+        const int m1 = k, m2 = 2 * k, m4 = 4 * k, m8 = 8 * k;   // < 4096
+        twiddle16_pow(v, ctw<S>(cmul(tw->h4k[m1 >> 6], tw->l4k[m1 & 63])),
+                      ctw<S>(cmul(tw->h4k[m2 >> 6], tw->l4k[m2 & 63])),
+                      ctw<S>(cmul(tw->h4k[m4 >> 6], tw->l4k[m4 & 63])),
+                      ctw<S>(cmul(tw->h4k[m8 >> 6], tw->l4k[m8 & 63])));
         dft16<S>(v);
         __syncthreads();
-        const int base = (t >> 8) * 4096 + k;
+        const int base = (t >> 8) * 4352 + k + (k >> 4);
+        float *wr = lre + base, *wi = lim + base;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            int o = lpad(base + 256 * r);
-            lre[o] = v[r].x;
-            lim[o] = v[r].y;
+            wr[272 * r] = v[r].x;
+            wi[272 * r] = v[r].y;
         }
         __syncthreads();
     }
     // pass 4: radix 4, Ns = 4096; butterfly j = t + 1024*b reads/writes j + 4096*r,
     // i.e. element t + 1024*(b + 4r) -> register v[b + 4r]
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        int i = lpad(t + 1024 * q);
-        v[q] = make_float2(lre[i], lim[i]);
-    }
+    for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const uint32_t j = (uint32_t)(t + 1024 * b);
-        const float2 w1 = tw1<S>(t16, j), w2 = tw1<S>(t16, 2u * j);
+        const uint32_t j2 = 2u * j;   // < 8192
+        const float2 w1 = ctw<S>(cmul(tw->h16k[j >> 7], tw->l16k[j & 127]));
+        const float2 w2 = ctw<S>(cmul(tw->h16k[j2 >> 7], tw->l16k[j2 & 127]));
         v[b + 4] = cmul(v[b + 4], w1);
         v[b + 8] = cmul(v[b + 8], w2);
         v[b + 12] = cmul(v[b + 12], cmul(w1, w2));

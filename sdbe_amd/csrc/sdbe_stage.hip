@@ -5,8 +5,8 @@
 //   k_fill_i32 + k_packet_map   packet index map [frame][cid][fid], first packet
 //                               wins; invalid / out-of-window packets counted
 //   k_reorder                   LDS transpose of 16 packets per (frame, cid pair)
-//                               into channel-contiguous 4-bit (re,im) rows,
-//                               missing packets -> code 2 (value 0), fid bitmap
+//                               into the reference int8 [snapshot][channel][im,re]
+//                               rows, missing packets -> 0 + 0j, fid bitmap
 // then per chunk of block-pols:
 //   k_stage_a    decode + C2R pre-processing of each 32768-point snapshot
 //                transform + 39-point DFT across the block's snapshots
@@ -39,7 +39,6 @@ namespace {
 constexpr int PKT_WORDS = 264;
 constexpr int NFR = 39;                       // frames per group
 constexpr int NSNAP_GROUP = NFR * 128;        // 4992 snapshots per group
-constexpr int ROW_WORDS = NCH / 8;            // 2048 words of 4-bit channels per snapshot
 constexpr int MAP_N = NFR * 256 * 8;          // 79,872 packet slots per group
 constexpr int MAP_NONE = 0x7FFFFFFF;
 constexpr int C1_TILES = 513;                 // 8193 column pairs / 16, rounded up
@@ -87,21 +86,21 @@ k_packet_map(const uint32_t *__restrict__ pk, int n, uint32_t bzero32, int *__re
 }
 
 // Workgroup (h = cid >> 1, frame): the 16 packets (fid 0..7, cid 2h, 2h+1)
-// cover channels [128h, 128h+128).  Output word w of a (pol, snapshot) row
-// holds channels 128h + 8w .. +7, 4 bits each: (re code << 2) | im code.
-// Payload byte q of a snapshot carries channel offset (3,2,1,0,7,6,5,4)[q];
-// its low nibble is pol 1, its high nibble pol 0
-// (vdif_inout_gpu_thread.cu:464-494).
+// cover channels [128h, 128h+128).  Payload byte q of a snapshot carries
+// channel offset (3,2,1,0,7,6,5,4)[q]; its bits from the LSB are pol1 im,
+// pol1 re, pol0 im, pol0 re, each decoded as (code & 3) - 2
+// (vdif_inout_gpu_thread.cu:387-392, 464-494).  Output: the reference int8
+// layout [pol][snapshot][16384][im, re] (its beng_0 / beng_1 buffers), a
+// missing packet leaving 0 + 0j; plus the fid bitmap and the unique count.
 __global__ void __launch_bounds__(256)
-k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, uint32_t *__restrict__ reord,
+k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, int8_t *__restrict__ beng,
           uint8_t *__restrict__ bitmap, int *__restrict__ cnt) {
     __shared__ int pidx[16];
-    __shared__ uint16_t half[2 * 128 * 16 * 2];   // [pol][snap][w][word half]
+    __shared__ uint32_t pay[16][257];     // packet payload words, [w][payload word]
     const int h = blockIdx.x, frame = blockIdx.y, t = threadIdx.x;
     if (t < 16) {
         const int w = t, fid = w >> 1, lo = w & 1;
-        const int v = map[(frame * 256 + 2 * h + lo) * 8 + fid];
-        pidx[w] = v;
+        pidx[w] = map[(frame * 256 + 2 * h + lo) * 8 + fid];
     }
     __syncthreads();
     if (t < 2) {   // fid bitmap byte of cid = 2h + t, and the unique count
@@ -117,101 +116,67 @@ k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, uint32_t
         if (found) atomicAdd(cnt + CNT_UNIQUE, found);
     }
 #pragma unroll 4
-    for (int it = 0; it < 16; ++it) {
-        const int e = it * 256 + t;
-        const int w = e >> 8, pw = e & 255, s = pw >> 1, hw = pw & 1;
-        const int p = pidx[w];
-        uint32_t val = 0xAAAAAAAAu;   // missing packet: every code 2 (value 0)
-        if (p != MAP_NONE) val = pk[(size_t)p * PKT_WORDS + 8 + pw];
-        const uint32_t b0 = val & 0xFFu, b1 = (val >> 8) & 0xFFu, b2 = (val >> 16) & 0xFFu,
-                       b3 = val >> 24;
-        const uint32_t p1 = (b3 & 0xFu) | ((b2 & 0xFu) << 4) | ((b1 & 0xFu) << 8) | ((b0 & 0xFu) << 12);
-        const uint32_t p0 = (b3 >> 4) | ((b2 >> 4) << 4) | ((b1 >> 4) << 8) | ((b0 >> 4) << 12);
-        half[(((0 * 128 + s) * 16 + w) << 1) + hw] = (uint16_t)p0;
-        half[(((1 * 128 + s) * 16 + w) << 1) + hw] = (uint16_t)p1;
+    for (int it = 0; it < 16; ++it) {      // coalesced 1 KB payload reads
+        const int p = pidx[it];
+        pay[it][t] = (p != MAP_NONE) ? pk[(size_t)p * PKT_WORDS + 8 + t] : 0xAAAAAAAAu;   // 0xAA: value 0
     }
     __syncthreads();
-    const uint32_t *h32 = reinterpret_cast<const uint32_t *>(half);
+    // thread -> (pol, snapshot, channel pair): 2 pols x 128 snapshots x 64 pairs
+    // = 16384 uint32 stores of (im, re, im, re) for two adjacent channels
 #pragma unroll 4
-    for (int it = 0; it < 16; ++it) {
+    for (int it = 0; it < 64; ++it) {
         const int e = it * 256 + t;
-        const int pol = e >> 11, s = (e >> 4) & 127, w = e & 15;
-        reord[((size_t)pol * NSNAP_GROUP + frame * 128 + s) * ROW_WORDS + h * 16 + w] = h32[e];
+        const int pol = e >> 13, sn = (e >> 6) & 127, cp = e & 63;
+        const int ch0 = 2 * cp;                 // channel within the 128-channel block
+        const int w = ch0 >> 3, off = ch0 & 7;  // packet slot, channel offset (even)
+        // offsets 0..3 sit in word 0 (bytes 3..0), 4..7 in word 1 (bytes 3..0)
+        const uint32_t word = pay[w][2 * sn + (off >> 2)];
+        const int q0 = 3 - (off & 3);           // byte of offset off; offset off+1 is byte q0-1
+        const uint32_t b0 = (word >> (8 * q0)) & 0xFFu, b1 = (word >> (8 * (q0 - 1))) & 0xFFu;
+        const int sh = pol == 0 ? 4 : 0;
+        const uint32_t n0 = (b0 >> sh) & 0xFu, n1 = (b1 >> sh) & 0xFu;
+        const uint32_t im0 = (uint32_t)(uint8_t)(int8_t)((int)(n0 & 3u) - 2);
+        const uint32_t re0 = (uint32_t)(uint8_t)(int8_t)((int)(n0 >> 2) - 2);
+        const uint32_t im1 = (uint32_t)(uint8_t)(int8_t)((int)(n1 & 3u) - 2);
+        const uint32_t re1 = (uint32_t)(uint8_t)(int8_t)((int)(n1 >> 2) - 2);
+        const size_t row = (size_t)pol * NSNAP_GROUP + (size_t)frame * 128 + sn;
+        uint32_t *dst = reinterpret_cast<uint32_t *>(beng + (row * NCH + (size_t)h * 128 + ch0) * 2);
+        *dst = im0 | (re0 << 8) | (im1 << 16) | (re1 << 24);
     }
-}
-
-// Expand the 4-bit rows into the reference int8 layout [snap][chan][im, re]
-// (the beng_0 / beng_1 buffers of vdif_inout_gpu_thread.cu:404-507).
-__global__ void __launch_bounds__(256)
-k_expand_int8(const uint32_t *__restrict__ reord, int8_t *__restrict__ b0, int8_t *__restrict__ b1) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // word index over both pols
-    const size_t per_pol = (size_t)NSNAP_GROUP * ROW_WORDS;
-    if (i >= 2 * per_pol) return;
-    const int pol = (int)(i / per_pol);
-    const size_t wi = i - (size_t)pol * per_pol;
-    const uint32_t w = reord[i];
-    uint32_t out[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        uint32_t acc = 0;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const uint32_t nib = (w >> (4 * (2 * q + c))) & 0xFu;
-            const uint32_t im = (uint32_t)(uint8_t)(int8_t)((int)(nib & 3u) - 2);
-            const uint32_t re = (uint32_t)(uint8_t)(int8_t)((int)((nib >> 2) & 3u) - 2);
-            acc |= (im | (re << 8)) << (16 * c);
-        }
-        out[q] = acc;
-    }
-    int8_t *dst = (pol == 0 ? b0 : b1) + wi * 16;
-    *reinterpret_cast<uint4 *>(dst) = make_uint4(out[0], out[1], out[2], out[3]);
 }
 
 // ---------------------------------------------------------------- resampler
 struct SrcDesc {
-    const uint32_t *nib;     // 4-bit rows [2][4992][2048] (group mode)
-    const int8_t *i8;        // int8 spectra [n_bp][39][16384][2] (spectra mode)
-    int group_mode;
+    const int8_t *i8;        // int8 [..][16384][2 = (im, re)] rows
+    int group_mode;          // 1: rows [2][4992] of a group, bp = 2 blk + pol
 };
-
-__device__ __forceinline__ float2 load_chan(const SrcDesc &sd, const uint32_t *row_nib,
-                                            const int8_t *row_i8, int k) {
-    if (sd.group_mode) {
-        const uint32_t w = row_nib[k >> 3];
-        const uint32_t nib = (w >> (4 * (k & 7))) & 0xFu;
-        return make_float2((float)((int)((nib >> 2) & 3u) - 2), (float)((int)(nib & 3u) - 2));
-    }
-    const char2 v = *reinterpret_cast<const char2 *>(row_i8 + 2 * k);
-    return make_float2((float)v.y, (float)v.x);
-}
 
 // Stage A: thread per channel k of a block-pol.  For each snapshot s the
 // packed C2R input Z_s[k] = (X[k] + conj(X[N2-k])) + i e^{2 pi i k/32768}
 // (X[k] - conj(X[N2-k])) (DC imaginary part and the zero Nyquist bin dropped,
 // as numpy irfft), then V[k1][k] = sum_s Z_s[k] e^{-2 pi i s k1/39}.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 k_stage_a(SrcDesc sd, int bp0, float2 *__restrict__ V, Tables tb) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int bpl = blockIdx.y, bp = bp0 + bpl;
-    const int km = NCH - k;   // 16384 for k == 0 -> zero Nyquist bin
-    const uint32_t *nib0 = nullptr;
-    const int8_t *i80 = nullptr;
+    size_t row0;
     if (sd.group_mode) {
         const int blk = bp >> 1, pol = bp & 1;
-        nib0 = sd.nib + ((size_t)pol * NSNAP_GROUP + (size_t)NBLK * blk) * ROW_WORDS;
+        row0 = (size_t)pol * NSNAP_GROUP + (size_t)NBLK * blk;
     } else {
-        i80 = sd.i8 + (size_t)bp * NBLK * NCH * 2;
+        row0 = (size_t)bp * NBLK;
     }
+    const char2 *src = reinterpret_cast<const char2 *>(sd.i8) + row0 * NCH;
+    const int km = (NCH - k) & (NCH - 1);      // k = 0 pairs with the zero Nyquist bin
+    const float kz = k == 0 ? 0.f : 1.f;
     const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
     float2 x[NBLK];
 #pragma unroll
     for (int s = 0; s < NBLK; ++s) {
-        const uint32_t *rn = nib0 ? nib0 + (size_t)s * ROW_WORDS : nullptr;
-        const int8_t *ri = i80 ? i80 + (size_t)s * NCH * 2 : nullptr;
-        float2 a = load_chan(sd, rn, ri, k);
-        float2 b = make_float2(0.f, 0.f);
-        if (km < NCH) b = load_chan(sd, rn, ri, km);
-        if (k == 0) a.y = 0.f;
+        const char2 va = src[(size_t)s * NCH + k];
+        const char2 vb = src[(size_t)s * NCH + km];
+        const float2 a = make_float2((float)va.y, kz * (float)va.x);     // DC imaginary part dropped
+        const float2 b = make_float2(kz * (float)vb.y, kz * (float)vb.x);  // Nyquist bin is zero
         const float2 sum = cadd(a, cconj(b));
         const float2 dif = csub(a, cconj(b));
         x[s] = cadd(sum, mul_si<+1>(cmul(T, dif)));
@@ -225,12 +190,15 @@ k_stage_a(SrcDesc sd, int bp0, float2 *__restrict__ V, Tables tb) {
 // Stage B: row k1 of block-pol bpl: w = IFFT16384(V[k1]); w[n] *= e^{-2 pi i n k1/638976};
 // Zf[k1 + 39 k2] = FFT16384(w)[k2]   (packed forward DFT of the 1,277,952 samples)
 // Workgroups are dispatched round-robin over the 8 XCDs; with xcd_map the 39
-// rows of a block-pol all run on one XCD so that the stride-39 stores of its
-// rows meet in that XCD's L2 (requires the chunk to be a multiple of 8).
+// rows of a block-pol all run on one XCD (requires the chunk to be a multiple
+// of 8).  The packed spectrum is stored row-major: Zf[k1 + 39 k2] at
+// k1 * 16384 + k2.
 __global__ void __launch_bounds__(1024)
 k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int xcd_map) {
     __shared__ float lre[LDS_PAD_N];
     __shared__ float lim[LDS_PAD_N];
+    __shared__ FftTw ftw;
+    fft_tw_load(&ftw, tb.fft);
     const int lin = blockIdx.x, t = threadIdx.x;
     int k1, bpl;
     if (xcd_map) {
@@ -245,15 +213,16 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = src[t + 1024 * q];
-    fft16k<+1>(v, lre, lim, tb.t16k);
+    fft16k<+1>(v, lre, lim, &ftw);
     // e^{-2 pi i (t + 1024 q) k1 / M'} = A(t) * B(q); B is uniform over the workgroup
     const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<-1>(tb.nmp, (uint32_t)(1024 * q * k1))));
-    fft16k<-1>(v, lre, lim, tb.t16k);
-    float2 *dst = Zf + (size_t)bpl * MP + k1;
+    fft16k<-1>(v, lre, lim, &ftw);
+    // row-major [k1][k2]: Zf[k1 + 39 k2] lives at k1 * 16384 + k2
+    float2 *dst = Zf + ((size_t)bpl * NBLK + k1) * NCH + t;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) dst[(size_t)NBLK * (t + 1024 * q)] = v[q];
+    for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
 }
 
 // Stage C1: workgroup = 16 column pairs (c, 16384 - c) of the [64][16384] view
@@ -265,6 +234,9 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
 // and the FP64 Parseval sum  sum_m w_m |Y[m]|^2 (w_0 = 1 on Re Y[0], else 2).
 // Twiddles factor per column and per row: e^{-2 pi i (c + 16384 r)/M} =
 // e^{-2 pi i c/M} e^{-2 pi i r/78}, e^{2 pi i (c + 16384 kh)/L} = e^{2 pi i c/L} e^{2 pi i kh/128}.
+// Grid: 8 XCD ranges x nb x 65 slots; XCD x handles tiles [65 x, 65 x + 65) of
+// every block-pol so that neighbouring tiles, which gather from the same
+// cache lines of the row-major packed spectrum, share that XCD's L2.
 __global__ void __launch_bounds__(256)
 k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__restrict__ part, Tables tb) {
     __shared__ float2 sz[32][41];
@@ -272,7 +244,10 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
     __shared__ float2 sc[32][65];
     __shared__ float2 colm[32], coll[32], rowm[40], rowl[64], w64[64];
     __shared__ double red[2][4];
-    const int tile = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
+    const int lin = blockIdx.x, t = threadIdx.x;
+    const int slot = lin >> 3, bpl = slot / 65;
+    const int tile = (lin & 7) * 65 + slot % 65;
+    if (tile >= C1_TILES) return;
     const float2 *zf = Zf + (size_t)bpl * MP;
     // column bookkeeping for column col: p = pair index, c = column
     auto col_c = [&](int col, int &c, bool &valid, bool &dup) {
@@ -303,7 +278,10 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
         col_c(col, c, valid, dup);
         const int m = c + NCH * r;
         float2 val = make_float2(0.f, 0.f);
-        if (valid && m <= MP) val = zf[m == MP ? 0 : m];
+        if (valid && m <= MP) {
+            const int mm = m == MP ? 0 : m;
+            val = zf[(mm % NBLK) * NCH + mm / NBLK];
+        }
         sz[col][r] = val;
     }
     __syncthreads();
@@ -485,6 +463,8 @@ k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp
            uint8_t *__restrict__ stage, Tables tb) {
     __shared__ float lre[LDS_PAD_N];
     __shared__ float lim[LDS_PAD_N];
+    __shared__ FftTw ftw;
+    fft_tw_load(&ftw, tb.fft);
     const int tx = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x, bp = bp0 + bpl;
     const float scale = 1.4551915228366852e-11f;   // 2^-36, exact
     const float2 *src = Bc + ((size_t)bpl * KHI + tx) * NCH;
@@ -496,7 +476,7 @@ k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp
     const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
-    fft16k<+1>(v, lre, lim, tb.t16k);
+    fft16k<+1>(v, lre, lim, &ftw);
     if (F32) {
         float2 *dst = reinterpret_cast<float2 *>(od.f32) + (size_t)bp * P20 + tx + KHI * t;
 #pragma unroll
@@ -595,11 +575,11 @@ __global__ void __launch_bounds__(256) k_headers(OutDesc od, HdrDesc hd) {
 
 // ======================================================================= host
 enum KernelId {
-    KID_FILL, KID_MAP, KID_REORDER, KID_EXPAND, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
+    KID_FILL, KID_MAP, KID_REORDER, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
     KID_PACK, KID_HEADERS, KID_N
 };
 static const char *const kKernelNames[KID_N] = {
-    "k_fill_i32", "k_packet_map", "k_reorder", "k_expand_int8", "k_stage_a", "k_stage_b",
+    "k_fill_i32", "k_packet_map", "k_reorder", "k_stage_a", "k_stage_b",
     "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers"};
 
 struct ProfRec {
@@ -619,7 +599,7 @@ struct sdbe_handle {
     float2 *tables = nullptr;
     Tables tb{};
     int *map = nullptr;
-    uint32_t *reord = nullptr;
+    int8_t *beng = nullptr;          // int8 rows [2][4992][16384][2] of the current group
     uint8_t *bitmap = nullptr;
     int *cnt = nullptr;
     float *latch = nullptr;
@@ -704,28 +684,40 @@ void tw_fill(float2 *host, const TwSpec &s) {
     }
 }
 
+void fft_tw_fill(float2 *host) {
+    const double two_pi = 6.283185307179586476925286766559;
+    auto e = [&](double num, double den) { return make_float2((float)cos(two_pi * num / den), (float)sin(two_pi * num / den)); };
+    float2 *p = host;
+    for (int m = 0; m < 256; ++m) *p++ = e(m, 256);
+    for (int m = 0; m < 64; ++m) *p++ = e(64.0 * m, 4096);
+    for (int m = 0; m < 64; ++m) *p++ = e(m, 4096);
+    for (int m = 0; m < 128; ++m) *p++ = e(128.0 * m, 16384);
+    for (int m = 0; m < 128; ++m) *p++ = e(m, 16384);
+}
+
 int build_tables(sdbe_handle_t *h) {
     const TwSpec specs[6] = {
-        {16384, 14, 0},                 // t16k: single level, 16384 entries
+        {16384, 14, 0},                 // unused slot (the FFT image is stored separately)
         {32768, 7, 16384},              // n32k
         {MP, 10, (int64_t)MP},          // nmp: n * k1 < 16384 * 39
         {2 * (int64_t)MP, 10, (int64_t)MP + 1},   // nm: m <= M'
         {2 * (int64_t)P20, 10, (int64_t)P20},     // nl: k < 2^20
         {P20, 10, (int64_t)P20},        // np: c * tx < 2^20, r * j * 16384 < 2^20
     };
-    size_t total = 0;
-    for (const auto &s : specs) total += tw_entries(s);
+    size_t total = FFT_TW_N;
+    for (int i = 1; i < 6; ++i) total += tw_entries(specs[i]);
     std::vector<float2> host(total);
     size_t off[6];
-    size_t o = 0;
-    for (int i = 0; i < 6; ++i) {
+    size_t o = FFT_TW_N;
+    fft_tw_fill(host.data());
+    for (int i = 1; i < 6; ++i) {
         off[i] = o;
         tw_fill(host.data() + o, specs[i]);
         o += tw_entries(specs[i]);
     }
     HIP_TRY(h, hipMalloc(&h->tables, total * sizeof(float2)));
     HIP_TRY(h, hipMemcpy(h->tables, host.data(), total * sizeof(float2), hipMemcpyHostToDevice));
-    h->tb.t16k = h->tables + off[0];
+    h->tb.fft = h->tables;
     Tw2 *dst[6] = {nullptr, &h->tb.n32k, &h->tb.nmp, &h->tb.nm, &h->tb.nl, &h->tb.np};
     for (int i = 1; i < 6; ++i) {
         dst[i]->lo = h->tables + off[i];
@@ -748,7 +740,7 @@ int run_resampler(sdbe_handle_t *h, const SrcDesc &sd, int n_bp, const OutDesc &
         LAUNCH(h, KID_A, nb, s, k_stage_a, dim3(NCH / 256, nb), dim3(256), sd, bp0, h->V, h->tb);
         LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
                (nb % 8) == 0 ? 1 : 0);
-        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(C1_TILES, nb), dim3(256), (const float2 *)h->Zf, h->Bc,
+        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(8 * 65 * nb), dim3(256), (const float2 *)h->Zf, h->Bc,
                h->part, h->tb);
         LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
         LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(64), (const double2 *)h->ms, nb, bp0, sdsc, h->latch,
@@ -774,7 +766,7 @@ int unpack_to_rows(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_
         LAUNCH(h, KID_MAP, 1, s, k_packet_map, dim3((n_pkts + 255) / 256), dim3(256), (const uint32_t *)d_pkts,
                n_pkts, (uint32_t)(b_zero & 0xFFFFFFFFull), h->map, h->cnt);
     LAUNCH(h, KID_REORDER, 1, s, k_reorder, dim3(128, NFR), dim3(256), (const uint32_t *)d_pkts,
-           (const int *)h->map, h->reord, d_bitmap ? d_bitmap : h->bitmap, h->cnt);
+           (const int *)h->map, h->beng, d_bitmap ? d_bitmap : h->bitmap, h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
 }
@@ -832,7 +824,7 @@ void sdbe_destroy(sdbe_handle_t *h) {
     (void)hipDeviceSynchronize();
     (void)hipFree(h->tables);
     (void)hipFree(h->map);
-    (void)hipFree(h->reord);
+    (void)hipFree(h->beng);
     (void)hipFree(h->bitmap);
     (void)hipFree(h->cnt);
     (void)hipFree(h->latch);
@@ -886,7 +878,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
         }                                              \
     } while (0)
     ALLOC(h->map, MAP_N * sizeof(int));
-    ALLOC(h->reord, 2 * (size_t)NSNAP_GROUP * ROW_WORDS * sizeof(uint32_t));
+    ALLOC(h->beng, 2 * (size_t)NSNAP_GROUP * NCH * 2);
     ALLOC(h->bitmap, NFR * 256);
     ALLOC(h->cnt, CNT_N * sizeof(int));
     ALLOC(h->latch, 2 * sizeof(float));
@@ -969,10 +961,9 @@ int sdbe_unpack_group_int8(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
     hipStream_t s = (hipStream_t)stream;
     int rc = unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;
-    const size_t words = 2 * (size_t)NSNAP_GROUP * ROW_WORDS;
-    LAUNCH(h, KID_EXPAND, 1, s, k_expand_int8, dim3((unsigned)((words + 255) / 256)), dim3(256),
-           (const uint32_t *)h->reord, d_beng0, d_beng1);
-    HIP_TRY(h, hipGetLastError());
+    const size_t pol_bytes = (size_t)NSNAP_GROUP * NCH * 2;
+    HIP_TRY(h, hipMemcpyAsync(d_beng0, h->beng, pol_bytes, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(d_beng1, h->beng + pol_bytes, pol_bytes, hipMemcpyDeviceToDevice, s));
     h->n_packets_last = n_pkts;
     h->n_bp_last = 0;
     h->have_group = 1;
@@ -989,7 +980,7 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     hipStream_t s = (hipStream_t)stream;
     int rc = unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;
-    SrcDesc sd = {h->reord, nullptr, 1};
+    SrcDesc sd = {h->beng, 1};
     OutDesc od;
     od.base = (uint8_t *)d_out;
     od.pol_stride = out_pol_stride_bytes;
@@ -1027,7 +1018,7 @@ int sdbe_process_spectra(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp, v
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
     const int frame_bytes = h->cfg.out_mode == SDBE_OUT_FRAMES ? OUT_FRAME_PAYLOAD + 32 : OUT_FRAME_PAYLOAD;
-    SrcDesc sd = {nullptr, d_beng, 0};
+    SrcDesc sd = {d_beng, 0};
     OutDesc od;
     od.base = (uint8_t *)d_out;
     od.pol_stride = 0;
@@ -1065,7 +1056,7 @@ int sdbe_resample_spectra_f32(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_
     if (n_bp > h->cfg.max_spectra_blockpols) return SDBE_ERR_CAPACITY;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    SrcDesc sd = {nullptr, d_beng, 0};
+    SrcDesc sd = {d_beng, 0};
     OutDesc od;
     memset(&od, 0, sizeof(od));
     od.f32 = d_samples;

@@ -41,7 +41,11 @@ constexpr int NFR = 39;                       // frames per group
 constexpr int NSNAP_GROUP = NFR * 128;        // 4992 snapshots per group
 constexpr int MAP_N = NFR * 256 * 8;          // 79,872 packet slots per group
 constexpr int MAP_NONE = 0x7FFFFFFF;
-constexpr int C1_TILES = 513;                 // 8193 column pairs / 16, rounded up
+// Stage C1 tiles: columns in residue classes mod 39, 16 per tile
+// (c = a + 39 (16 u + j)), 14 tiles per class, 39 classes.
+constexpr int C1_CLASS_TILES = 14;
+constexpr int C1_TILES = NBLK * C1_CLASS_TILES;   // 546
+constexpr int BC_ROW = C1_TILES * 32;            // 17472 slots per row of the first-pass output
 constexpr int OUT_FRAME_PAYLOAD = 8192;
 constexpr int OUT_FRAMES_BLOCK = 64;
 
@@ -232,34 +236,40 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
 //   Bc[tx][c] = sum_kh Zc[c + 16384 kh] e^{2 pi i kh tx/64}
 // (the inter-pass twiddle e^{2 pi i c tx/P} is applied by stage C2 on load)
 // and the FP64 Parseval sum  sum_m w_m |Y[m]|^2 (w_0 = 1 on Re Y[0], else 2).
-// Twiddles factor per column and per row: e^{-2 pi i (c + 16384 r)/M} =
-// e^{-2 pi i c/M} e^{-2 pi i r/78}, e^{2 pi i (c + 16384 kh)/L} = e^{2 pi i c/L} e^{2 pi i kh/128}.
-// Grid: 8 XCD ranges x nb x 65 slots; XCD x handles tiles [65 x, 65 x + 65) of
-// every block-pol so that neighbouring tiles, which gather from the same
-// cache lines of the row-major packed spectrum, share that XCD's L2.
+// Tile (a, u) takes the lower columns c = a + 39 (16 u + j), j < 16: for each
+// row r the 16 values m = c + 16384 r share the packed-spectrum row
+// (m mod 39 = (a + 4 r) mod 39) at consecutive k2, so every read is a
+// 128-byte run of the row-major store; the mirror columns likewise.  Output
+// slots are tile-major (slot = 32 tile + col) and stage C2 puts them back in
+// column order.  Twiddles factor per column and per row:
+// e^{-2 pi i (c + 16384 r)/M} = e^{-2 pi i c/M} e^{-2 pi i r/78},
+// e^{2 pi i (c + 16384 kh)/L} = e^{2 pi i c/L} e^{2 pi i kh/128}.
+__device__ __forceinline__ void c1_column(int tile, int col, int &c, bool &valid, bool &dup) {
+    const int a = tile / C1_CLASS_TILES, u = tile - a * C1_CLASS_TILES;
+    const int cl = a + NBLK * (16 * u + (col & 15));
+    valid = cl <= NCH / 2;
+    c = col < 16 ? cl : ((NCH - cl) & (NCH - 1));
+    dup = (col >= 16) && (cl == 0 || cl == NCH / 2);
+}
+
 __global__ void __launch_bounds__(256)
 k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__restrict__ part, Tables tb) {
     __shared__ float2 sz[32][41];
     __shared__ float2 sy[32][41];
     __shared__ float2 sc[32][65];
     __shared__ float2 colm[32], coll[32], rowm[40], rowl[64], w64[64];
+    __shared__ int colc[32];
+    __shared__ uint8_t colv[32], cold[32];
     __shared__ double red[2][4];
-    const int lin = blockIdx.x, t = threadIdx.x;
-    const int slot = lin >> 3, bpl = slot / 65;
-    const int tile = (lin & 7) * 65 + slot % 65;
-    if (tile >= C1_TILES) return;
+    const int tile = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
     const float2 *zf = Zf + (size_t)bpl * MP;
-    // column bookkeeping for column col: p = pair index, c = column
-    auto col_c = [&](int col, int &c, bool &valid, bool &dup) {
-        const int p = tile * 16 + (col & 15);
-        valid = p <= NCH / 2;
-        c = col < 16 ? p : ((NCH - p) & (NCH - 1));
-        dup = (col >= 16) && (p == 0 || p == NCH / 2);
-    };
     if (t < 32) {
         int c;
         bool valid, dup;
-        col_c(t, c, valid, dup);
+        c1_column(tile, t, c, valid, dup);
+        colc[t] = c;
+        colv[t] = valid;
+        cold[t] = dup;
         colm[t] = tw<-1>(tb.nm, (uint32_t)c);
         coll[t] = tw<+1>(tb.nl, (uint32_t)c);
     } else if (t < 72) {
@@ -270,31 +280,33 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
         const int e = t - 136;   // W64^(r j) for r = e >> 3, j = e & 7
         w64[e] = tw<+1>(tb.np, (uint32_t)((e >> 3) * (e & 7) * NCH));
     }
-    // load Zf rows 0..39
-    for (int e = t; e < 40 * 32; e += 256) {
+    __syncthreads();
+    // load Zf rows 0..39 (row-major packed spectrum: m -> (m mod 39) * 16384 + m / 39)
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const int e = it * 256 + t;
         const int r = e >> 5, col = e & 31;
-        int c;
-        bool valid, dup;
-        col_c(col, c, valid, dup);
+        const int c = colc[col];
         const int m = c + NCH * r;
         float2 val = make_float2(0.f, 0.f);
-        if (valid && m <= MP) {
+        if (colv[col] && m <= MP) {
             const int mm = m == MP ? 0 : m;
-            val = zf[(mm % NBLK) * NCH + mm / NBLK];
+            const int k1 = mm % NBLK;
+            val = zf[k1 * NCH + (mm - k1) / NBLK];
         }
         sz[col][r] = val;
     }
     __syncthreads();
     // Y rows 0..39 (zero above M') + Parseval
     double acc = 0.0, dc = 0.0;
-    for (int e = t; e < 40 * 32; e += 256) {
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const int e = it * 256 + t;
         const int r = e >> 5, col = e & 31;
-        int c;
-        bool valid, dup;
-        col_c(col, c, valid, dup);
+        const int c = colc[col];
         const int m = c + NCH * r;
         float2 Y = make_float2(0.f, 0.f);
-        if (valid && m <= MP) {
+        if (colv[col] && m <= MP) {
             const int pcol = col ^ 16;
             const int pr = (c == 0) ? (39 - r) : (38 - r);
             const float2 zm = sz[col][r];
@@ -302,7 +314,7 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
             const float2 E = cscale(cadd(zm, zp), 0.5f);
             const float2 D = cscale(csub(zm, zp), 0.5f);
             Y = cadd(E, mul_si<-1>(cmul(cmul(colm[col], rowm[r]), D)));
-            if (!dup) {
+            if (!cold[col]) {
                 if (m == 0) {
                     acc += (double)Y.x * (double)Y.x;
                     dc = (double)Y.x;
@@ -315,9 +327,11 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
     }
     __syncthreads();
     // Zc rows 0..63
-    for (int e = t; e < 64 * 32; e += 256) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int e = it * 256 + t;
         const int kh = e >> 5, col = e & 31;
-        const int c = col < 16 ? tile * 16 + col : ((NCH - (tile * 16 + (col & 15))) & (NCH - 1));
+        const int c = colc[col];
         const int pcol = col ^ 16;
         const int pr = (c == 0) ? (64 - kh) : (63 - kh);
         const float2 ck = kh <= 39 ? sy[col][kh] : make_float2(0.f, 0.f);
@@ -326,8 +340,9 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
     }
     __syncthreads();
     // 64-point inverse DFT over kh per column: 8 threads x radix 8, two passes
+    // (lanes of a half-wave take different columns: odd row stride, no bank conflicts)
     {
-        const int col = t >> 3, j = t & 7;
+        const int col = t & 31, j = t >> 5;
         float2 v[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
@@ -346,14 +361,13 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
         for (int r = 0; r < 8; ++r) sc[col][j + 8 * r] = v[r];
     }
     __syncthreads();
-    // write Bc[tx][c]: half-waves of 16 consecutive (lower) / reversed (upper) columns
-    float2 *bc = Bc + (size_t)bpl * P20;
-    for (int e = t; e < 64 * 32; e += 256) {
+    // write the tile's 32 slots of every row tx (256-byte runs)
+    float2 *bc = Bc + (size_t)bpl * KHI * BC_ROW + (size_t)tile * 32;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int e = it * 256 + t;
         const int tx = e >> 5, col = e & 31;
-        int c;
-        bool valid, dup;
-        col_c(col, c, valid, dup);
-        if (valid && !dup) bc[(size_t)tx * NCH + c] = sc[col][tx];
+        if (colv[col] && !cold[col]) bc[(size_t)tx * BC_ROW + col] = sc[col][tx];
     }
     // Parseval partial: wave reduce then cross-wave
 #pragma unroll
@@ -460,17 +474,53 @@ struct OutDesc {
 template <bool F32>
 __global__ void __launch_bounds__(1024)
 k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp0, OutDesc od,
-           uint8_t *__restrict__ stage, Tables tb) {
+           uint8_t *__restrict__ stage, const int *__restrict__ slot_col, Tables tb) {
     __shared__ float lre[LDS_PAD_N];
     __shared__ float lim[LDS_PAD_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, tb.fft);
     const int tx = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x, bp = bp0 + bpl;
     const float scale = 1.4551915228366852e-11f;   // 2^-36, exact
-    const float2 *src = Bc + ((size_t)bpl * KHI + tx) * NCH;
-    float2 v[16];
+    // the row arrives in stage C1's tile-major slot order: scatter it into LDS in
+    // column order (slot -> column table, -1 for slots without a column)
+    // (all loads are issued before any LDS store; slots without a column are
+    // read too -- the row is allocated -- and dropped)
+    const float2 *src = Bc + ((size_t)bpl * KHI + tx) * BC_ROW;
+    constexpr int NSLOT = BC_ROW / 1024;          // 17 full passes ...
+    constexpr int REM = BC_ROW - NSLOT * 1024;    // ... and 64 slots left
+    {
+        float2 val[NSLOT];
+        int cc[NSLOT];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = src[t + 1024 * q];
+        for (int i = 0; i < NSLOT; ++i) {
+            val[i] = src[t + 1024 * i];
+            cc[i] = slot_col[t + 1024 * i];
+        }
+        float2 vr = make_float2(0.f, 0.f);
+        int cr = -1;
+        if (t < REM) {
+            vr = src[t + 1024 * NSLOT];
+            cr = slot_col[t + 1024 * NSLOT];
+        }
+#pragma unroll
+        for (int i = 0; i < NSLOT; ++i) {
+            if (cc[i] >= 0) {
+                lre[cc[i] + (cc[i] >> 4)] = val[i].x;
+                lim[cc[i] + (cc[i] >> 4)] = val[i].y;
+            }
+        }
+        if (cr >= 0) {
+            lre[cr + (cr >> 4)] = vr.x;
+            lim[cr + (cr >> 4)] = vr.y;
+        }
+    }
+    __syncthreads();
+    float2 v[16];
+    {
+        const int rd = t + (t >> 4);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
+    }
     // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
     // times a workgroup-uniform one
     const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
@@ -608,6 +658,7 @@ struct sdbe_handle {
     float2 *thr = nullptr;
     double2 *ms = nullptr;
     uint8_t *stage = nullptr;
+    int *slot_col = nullptr;         // stage C1 output slot -> column (or -1)
     // pinned host mirrors
     int *h_cnt = nullptr;
     float *h_latch = nullptr;
@@ -727,6 +778,27 @@ int build_tables(sdbe_handle_t *h) {
     return SDBE_OK;
 }
 
+// Slot -> column table of the stage C1 output (same rule as c1_column); every
+// column 0..16383 must own exactly one slot.
+int build_slot_table(sdbe_handle_t *h) {
+    std::vector<int> tab(BC_ROW, -1), seen(NCH, 0);
+    for (int tile = 0; tile < C1_TILES; ++tile) {
+        const int a = tile / C1_CLASS_TILES, u = tile - a * C1_CLASS_TILES;
+        for (int col = 0; col < 32; ++col) {
+            const int cl = a + NBLK * (16 * u + (col & 15));
+            if (cl > NCH / 2) continue;
+            if (col >= 16 && (cl == 0 || cl == NCH / 2)) continue;
+            const int c = col < 16 ? cl : ((NCH - cl) & (NCH - 1));
+            tab[tile * 32 + col] = c;
+            seen[c]++;
+        }
+    }
+    for (int c = 0; c < NCH; ++c)
+        if (seen[c] != 1) return SDBE_ERR_STATE;
+    HIP_TRY(h, hipMemcpy(h->slot_col, tab.data(), BC_ROW * sizeof(int), hipMemcpyHostToDevice));
+    return SDBE_OK;
+}
+
 // Run stages A..C2 over block-pols [0, n_bp) in chunks.
 int run_resampler(sdbe_handle_t *h, const SrcDesc &sd, int n_bp, const OutDesc &od,
                   double *d_stats, hipStream_t s) {
@@ -740,17 +812,17 @@ int run_resampler(sdbe_handle_t *h, const SrcDesc &sd, int n_bp, const OutDesc &
         LAUNCH(h, KID_A, nb, s, k_stage_a, dim3(NCH / 256, nb), dim3(256), sd, bp0, h->V, h->tb);
         LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
                (nb % 8) == 0 ? 1 : 0);
-        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(8 * 65 * nb), dim3(256), (const float2 *)h->Zf, h->Bc,
+        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(C1_TILES, nb), dim3(256), (const float2 *)h->Zf, h->Bc,
                h->part, h->tb);
         LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
         LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(64), (const double2 *)h->ms, nb, bp0, sdsc, h->latch,
                h->thr);
         if (od.f32) {
             LAUNCH(h, KID_C2, nb, s, k_stage_c2<true>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
-                   (const float2 *)h->thr, bp0, od, h->stage, h->tb);
+                   (const float2 *)h->thr, bp0, od, h->stage, (const int *)h->slot_col, h->tb);
         } else {
             LAUNCH(h, KID_C2, nb, s, k_stage_c2<false>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
-                   (const float2 *)h->thr, bp0, od, h->stage, h->tb);
+                   (const float2 *)h->thr, bp0, od, h->stage, (const int *)h->slot_col, h->tb);
             LAUNCH(h, KID_PACK, nb, s, k_pack, dim3(NCH / 64, nb), dim3(256), (const uint8_t *)h->stage, bp0, od);
         }
         HIP_TRY(h, hipGetLastError());
@@ -834,6 +906,7 @@ void sdbe_destroy(sdbe_handle_t *h) {
     (void)hipFree(h->part);
     (void)hipFree(h->thr);
     (void)hipFree(h->stage);
+    (void)hipFree(h->slot_col);
     (void)hipFree(h->ms);
     (void)hipHostFree(h->h_cnt);
     (void)hipHostFree(h->h_latch);
@@ -884,7 +957,8 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->latch, 2 * sizeof(float));
     ALLOC(h->V, C * NBLK * NCH * sizeof(float2));
     ALLOC(h->Zf, C * MP * sizeof(float2));
-    ALLOC(h->Bc, C * P20 * sizeof(float2));
+    ALLOC(h->Bc, C * KHI * BC_ROW * sizeof(float2));
+    ALLOC(h->slot_col, BC_ROW * sizeof(int));
     ALLOC(h->part, C * C1_TILES * 2 * sizeof(double));
     ALLOC(h->thr, C * sizeof(float2));
     ALLOC(h->stage, C * KHI * NCH);
@@ -898,6 +972,8 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     memset(h->h_cnt, 0, CNT_N * sizeof(int));
     memset(h->h_latch, 0, 2 * sizeof(float));
     rc = build_tables(h);
+    if (rc != SDBE_OK) goto fail;
+    rc = build_slot_table(h);
     if (rc != SDBE_OK) goto fail;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, h->cnt, CNT_N, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, (int *)h->latch, 2, 0);

@@ -33,8 +33,8 @@ HBM_PEAK = 8.0e12
 # once plus outputs written once.
 ALG_BYTES = {
     "k_packet_map": 79872 * 32,                          # header sector per packet
-    "k_reorder": 79872 * 1024 + 2 * 4992 * 8192,         # payloads in, 4-bit rows out
-    "k_stage_a": 39 * 16384 // 2 + 39 * 16384 * 8,       # 4-bit rows in, V out
+    "k_reorder": 79872 * 1024 + 2 * 4992 * 32768,        # payloads in, int8 (im,re) rows out
+    "k_stage_a": 39 * 16384 * 2 + 39 * 16384 * 8,        # int8 rows in, V out
     "k_stage_b": 2 * 39 * 16384 * 8,                     # V in, packed R2C out
     "k_stage_c1": 39 * 16384 * 8 + (1 << 20) * 8,        # packed R2C in, first C2R pass out
     "k_stage_c2": (1 << 20) * 8 + (1 << 20),             # second pass in, 4-bit codes out
@@ -45,6 +45,22 @@ ALG_BYTES = {
 }
 B_ALG_PER_BP = 48304512          # stage-boundary algorithmic bytes per block-pol
 COMPULSORY_PER_BP = 329472 + 526336
+
+
+def measured_traffic(kernel, units_per_launch, chunk):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/traffic.json, counters scaled by a plain-copy
+    calibration), or None when that summary was taken at another chunk size or
+    does not list the kernel."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if k is None or t.get("chunk") != chunk:
+        return None
+    return int(round(k["bytes_per_unit"] * units_per_launch))
 
 
 def log(*a):
@@ -194,7 +210,7 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved * 1e9 / HBM_PEAK, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dname, du / dl, args.chunk),
         },
         "pipeline": {
             "block_pols_per_s": round(bp_per_s, 1),

@@ -868,7 +868,7 @@ int sdbe_default_config(sdbe_config_t *cfg) {
     cfg->fixed_threshold = 1.0f;
     cfg->fixed_offset = 0.0f;
     cfg->out_mode = SDBE_OUT_PAYLOAD;
-    cfg->chunk_blockpols = 32;
+    cfg->chunk_blockpols = 128;
     cfg->max_spectra_blockpols = 256;
     cfg->station_id = (uint16_t)((('X' & 0xFF) << 8) | ('X' & 0xFF));
     return SDBE_OK;

@@ -59,6 +59,7 @@ struct Tables {
     Tw2 nm;      // N = 1,277,952    (R2C post-processing)
     Tw2 nl;      // N = 2,097,152    (C2R pre-processing of the final stage)
     Tw2 np;      // N = 1,048,576    (inter-pass twiddle of the final stage)
+    const float2 *c1col;  // [2][16384]: e^{-2 pi i c/1,277,952}, e^{2 pi i c/2,097,152}
 };
 
 // ---------------------------------------------------------------- small DFTs

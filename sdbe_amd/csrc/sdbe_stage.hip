@@ -45,6 +45,7 @@ constexpr int MAP_NONE = 0x7FFFFFFF;
 // (c = a + 39 (16 u + j)), 14 tiles per class, 39 classes.
 constexpr int C1_CLASS_TILES = 14;
 constexpr int C1_TILES = NBLK * C1_CLASS_TILES;   // 546
+constexpr int C1_TILES_PER_WG = 8;                 // consecutive tiles run by one stage C1 workgroup
 constexpr int BC_ROW = C1_TILES * 32;            // 17472 slots per row of the first-pass output
 constexpr int OUT_FRAME_PAYLOAD = 8192;
 constexpr int OUT_FRAMES_BLOCK = 64;
@@ -150,6 +151,14 @@ k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, int8_t *
 }
 
 // ---------------------------------------------------------------- resampler
+// Workgroup `lin` of a 1D grid of `total` runs on XCD lin % 8 (round-robin
+// dispatch).  The returned logical index is contiguous per XCD: logical
+// neighbours share that XCD's L2 and are dispatched close together in time.
+__device__ __forceinline__ int xcd_contig(int lin, int total) {
+    const int x = lin & 7, slot = lin >> 3, q = total >> 3, r = total & 7;
+    return x * q + (x < r ? x : r) + slot;
+}
+
 struct SrcDesc {
     const int8_t *i8;        // int8 [..][16384][2 = (im, re)] rows
     int group_mode;          // 1: rows [2][4992] of a group, bp = 2 blk + pol
@@ -159,10 +168,16 @@ struct SrcDesc {
 // packed C2R input Z_s[k] = (X[k] + conj(X[N2-k])) + i e^{2 pi i k/32768}
 // (X[k] - conj(X[N2-k])) (DC imaginary part and the zero Nyquist bin dropped,
 // as numpy irfft), then V[k1][k] = sum_s Z_s[k] e^{-2 pi i s k1/39}.
+// Workgroups are paired (w, 63 - w) on one XCD back to back, so the mirror
+// channels N2 - k a workgroup reads are the ones its partner reads as k and
+// are served from that XCD's L2.
 __global__ void __launch_bounds__(256, 2)
-k_stage_a(SrcDesc sd, int bp0, float2 *__restrict__ V, Tables tb) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    const int bpl = blockIdx.y, bp = bp0 + bpl;
+k_stage_a(SrcDesc sd, int bp0, int nb, float2 *__restrict__ V, Tables tb) {
+    const int L = xcd_contig((int)blockIdx.x, (NCH / 256) * nb);
+    const int bpl = L >> 6, idx = L & 63;
+    const int w = (idx & 1) ? 63 - (idx >> 1) : (idx >> 1);
+    const int k = w * 256 + threadIdx.x;
+    const int bp = bp0 + bpl;
     size_t row0;
     if (sd.group_mode) {
         const int blk = bp >> 1, pol = bp & 1;
@@ -252,138 +267,163 @@ __device__ __forceinline__ void c1_column(int tile, int col, int &c, bool &valid
     dup = (col >= 16) && (cl == 0 || cl == NCH / 2);
 }
 
+// Each workgroup runs `tpw` consecutive tiles in (block-pol, a, u) order, and
+// the workgroups are contiguous per XCD: tiles u and u + 1 of a class share the
+// 128-byte lines at the ends of their runs, which the second finds in L2.  The
+// next tile's packed-spectrum values and column twiddles are loaded while the
+// current tile is computed.
+struct C1Tile {
+    int bpl, tile, c;
+    bool valid, dup;
+};
+
+__device__ __forceinline__ void c1_fetch(const float2 *__restrict__ Zf, const Tables &tb, int lt, int col, int rw,
+                                         C1Tile &x, float2 (&z)[5], float2 &cm, float2 &cl) {
+    x.bpl = lt / C1_TILES;
+    x.tile = lt - x.bpl * C1_TILES;
+    c1_column(x.tile, col, x.c, x.valid, x.dup);
+    const int a = x.tile / C1_CLASS_TILES, u = x.tile - a * C1_CLASS_TILES, j16 = col & 15;
+    const bool mirror = col >= 16 && x.c != 0;
+    const float2 *zf = Zf + (size_t)x.bpl * MP;
+    // m = a + 16384 r + 39 (16 u + j) sits at row (a + 16384 r) mod 39, column
+    // (a + 16384 r) / 39 + 16 u + j of the row-major store; the mirror column
+    // 16384 - c at m = 16384 (r + 1) - a - 39 (16 u + j).  Row 39 holds only
+    // m = M' (c = 0), which the store keeps at address 0 (as m = 0); values
+    // that are not used also read address 0, so the loads are unconditional and
+    // all five are in flight together.
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const int r = it * 8 + rw;
+        const bool ok = x.valid && r < NBLK;
+        int addr;
+        if (mirror) {
+            const int hi = NCH * (r + 1) - a;
+            addr = (hi % NBLK) * NCH + hi / NBLK - 16 * u - j16;
+        } else {
+            const int lo = a + NCH * r;
+            addr = (lo % NBLK) * NCH + lo / NBLK + 16 * u + j16;
+        }
+        z[it] = zf[ok ? addr : 0];
+    }
+    cm = tb.c1col[x.c];
+    cl = tb.c1col[NCH + x.c];
+}
+
 __global__ void __launch_bounds__(256)
-k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__restrict__ part, Tables tb) {
+k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__restrict__ part, Tables tb, int nb,
+           int tpw) {
     __shared__ float2 sz[32][41];
     __shared__ float2 sy[32][41];
     __shared__ float2 sc[32][65];
-    __shared__ float2 colm[32], coll[32], rowm[40], rowl[64], w64[64];
-    __shared__ int colc[32];
-    __shared__ uint8_t colv[32], cold[32];
+    __shared__ float2 rowm[40], rowl[64], w64[64];
     __shared__ double red[2][4];
-    const int tile = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
-    const float2 *zf = Zf + (size_t)bpl * MP;
-    if (t < 32) {
-        int c;
-        bool valid, dup;
-        c1_column(tile, t, c, valid, dup);
-        colc[t] = c;
-        colv[t] = valid;
-        cold[t] = dup;
-        colm[t] = tw<-1>(tb.nm, (uint32_t)c);
-        coll[t] = tw<+1>(tb.nl, (uint32_t)c);
-    } else if (t < 72) {
-        rowm[t - 32] = tw<-1>(tb.nm, (uint32_t)(NCH * (t - 32)));
-    } else if (t < 136) {
-        rowl[t - 72] = tw<+1>(tb.nl, (uint32_t)(NCH * (t - 72)));
-    } else if (t < 200) {
-        const int e = t - 136;   // W64^(r j) for r = e >> 3, j = e & 7
+    const int t = threadIdx.x, col = t & 31, rw = t >> 5, pcol = col ^ 16;
+    const int n_tiles = C1_TILES * nb;
+    const int first = xcd_contig((int)blockIdx.x, (int)gridDim.x) * tpw;
+    const int last = min(first + tpw, n_tiles);
+    if (t < 40) {
+        rowm[t] = tw<-1>(tb.nm, (uint32_t)(NCH * t));
+    } else if (t < 104) {
+        rowl[t - 40] = tw<+1>(tb.nl, (uint32_t)(NCH * (t - 40)));
+    } else if (t < 168) {
+        const int e = t - 104;   // W64^(r j) for r = e >> 3, j = e & 7
         w64[e] = tw<+1>(tb.np, (uint32_t)((e >> 3) * (e & 7) * NCH));
     }
-    __syncthreads();
-    // load Zf rows 0..39 (row-major packed spectrum: m -> (m mod 39) * 16384 + m / 39)
+    C1Tile cur;
+    float2 z[5], cm, cl;
+    if (first < last) c1_fetch(Zf, tb, first, col, rw, cur, z, cm, cl);
+    for (int lt = first; lt < last; ++lt) {
+        const C1Tile x = cur;
+        const int c = x.c;
+        const float2 cmc = cm, clc = cl;
 #pragma unroll
-    for (int it = 0; it < 5; ++it) {
-        const int e = it * 256 + t;
-        const int r = e >> 5, col = e & 31;
-        const int c = colc[col];
-        const int m = c + NCH * r;
-        float2 val = make_float2(0.f, 0.f);
-        if (colv[col] && m <= MP) {
-            const int mm = m == MP ? 0 : m;
-            const int k1 = mm % NBLK;
-            val = zf[k1 * NCH + (mm - k1) / NBLK];
+        for (int it = 0; it < 5; ++it) {
+            const int r = it * 8 + rw;
+            const bool ok = x.valid && (r < NBLK || c == 0);
+            sz[col][r] = ok ? z[it] : make_float2(0.f, 0.f);
         }
-        sz[col][r] = val;
-    }
-    __syncthreads();
-    // Y rows 0..39 (zero above M') + Parseval
-    double acc = 0.0, dc = 0.0;
+        __syncthreads();
+        if (lt + 1 < last) c1_fetch(Zf, tb, lt + 1, col, rw, cur, z, cm, cl);
+        // Y rows 0..39 (zero above M') + Parseval
+        double acc = 0.0, dc = 0.0;
 #pragma unroll
-    for (int it = 0; it < 5; ++it) {
-        const int e = it * 256 + t;
-        const int r = e >> 5, col = e & 31;
-        const int c = colc[col];
-        const int m = c + NCH * r;
-        float2 Y = make_float2(0.f, 0.f);
-        if (colv[col] && m <= MP) {
-            const int pcol = col ^ 16;
-            const int pr = (c == 0) ? (39 - r) : (38 - r);
-            const float2 zm = sz[col][r];
-            const float2 zp = cconj(sz[pcol][pr]);
-            const float2 E = cscale(cadd(zm, zp), 0.5f);
-            const float2 D = cscale(csub(zm, zp), 0.5f);
-            Y = cadd(E, mul_si<-1>(cmul(cmul(colm[col], rowm[r]), D)));
-            if (!cold[col]) {
-                if (m == 0) {
-                    acc += (double)Y.x * (double)Y.x;
-                    dc = (double)Y.x;
-                } else {
-                    acc += 2.0 * ((double)Y.x * (double)Y.x + (double)Y.y * (double)Y.y);
+        for (int it = 0; it < 5; ++it) {
+            const int r = it * 8 + rw;
+            float2 Y = make_float2(0.f, 0.f);
+            if (x.valid && (r < NBLK || c == 0)) {
+                const int pr = (c == 0) ? (39 - r) : (38 - r);
+                const float2 zm = sz[col][r];
+                const float2 zp = cconj(sz[pcol][pr]);
+                const float2 E = cscale(cadd(zm, zp), 0.5f);
+                const float2 D = cscale(csub(zm, zp), 0.5f);
+                Y = cadd(E, mul_si<-1>(cmul(cmul(cmc, rowm[r]), D)));
+                if (!x.dup) {
+                    if (c == 0 && r == 0) {
+                        acc += (double)Y.x * (double)Y.x;
+                        dc = (double)Y.x;
+                    } else {
+                        acc += 2.0 * ((double)Y.x * (double)Y.x + (double)Y.y * (double)Y.y);
+                    }
                 }
             }
+            sy[col][r] = Y;
         }
-        sy[col][r] = Y;
-    }
-    __syncthreads();
-    // Zc rows 0..63
-#pragma unroll
-    for (int it = 0; it < 8; ++it) {
-        const int e = it * 256 + t;
-        const int kh = e >> 5, col = e & 31;
-        const int c = colc[col];
-        const int pcol = col ^ 16;
-        const int pr = (c == 0) ? (64 - kh) : (63 - kh);
-        const float2 ck = kh <= 39 ? sy[col][kh] : make_float2(0.f, 0.f);
-        const float2 cp = pr <= 39 ? cconj(sy[pcol][pr]) : make_float2(0.f, 0.f);
-        sc[col][kh] = cadd(cadd(ck, cp), mul_si<+1>(cmul(cmul(coll[col], rowl[kh]), csub(ck, cp))));
-    }
-    __syncthreads();
-    // 64-point inverse DFT over kh per column: 8 threads x radix 8, two passes
-    // (lanes of a half-wave take different columns: odd row stride, no bank conflicts)
-    {
-        const int col = t & 31, j = t >> 5;
-        float2 v[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
-        dft8<+1>(v);
         __syncthreads();
+        // Zc rows 0..63
 #pragma unroll
-        for (int r = 0; r < 8; ++r) sc[col][8 * j + r] = v[r];
+        for (int it = 0; it < 8; ++it) {
+            const int kh = it * 8 + rw;
+            const int pr = (c == 0) ? (64 - kh) : (63 - kh);
+            const float2 ck = kh <= 39 ? sy[col][kh] : make_float2(0.f, 0.f);
+            const float2 cp = pr <= 39 ? cconj(sy[pcol][pr]) : make_float2(0.f, 0.f);
+            sc[col][kh] = cadd(cadd(ck, cp), mul_si<+1>(cmul(cmul(clc, rowl[kh]), csub(ck, cp))));
+        }
         __syncthreads();
+        // 64-point inverse DFT over kh per column: 8 threads x radix 8, two passes
+        // (lanes of a half-wave take different columns: odd row stride, no bank conflicts)
+        {
+            const int j = rw;
+            float2 v[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
+            for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
+            dft8<+1>(v);
+            __syncthreads();
 #pragma unroll
-        for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], w64[8 * r + j]);
-        dft8<+1>(v);
+            for (int r = 0; r < 8; ++r) sc[col][8 * j + r] = v[r];
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
+#pragma unroll
+            for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], w64[8 * r + j]);
+            dft8<+1>(v);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) sc[col][j + 8 * r] = v[r];
+        }
         __syncthreads();
+        // write the tile's 32 slots of every row tx (256-byte runs)
+        float2 *bc = Bc + (size_t)x.bpl * KHI * BC_ROW + (size_t)x.tile * 32;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) sc[col][j + 8 * r] = v[r];
-    }
-    __syncthreads();
-    // write the tile's 32 slots of every row tx (256-byte runs)
-    float2 *bc = Bc + (size_t)bpl * KHI * BC_ROW + (size_t)tile * 32;
+        for (int it = 0; it < 8; ++it) {
+            const int tx = it * 8 + rw;
+            if (x.valid && !x.dup) bc[(size_t)tx * BC_ROW + col] = sc[col][tx];
+        }
+        // Parseval partial: wave reduce then cross-wave
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {
-        const int e = it * 256 + t;
-        const int tx = e >> 5, col = e & 31;
-        if (colv[col] && !cold[col]) bc[(size_t)tx * BC_ROW + col] = sc[col][tx];
-    }
-    // Parseval partial: wave reduce then cross-wave
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        acc += __shfl_xor(acc, off);
-        dc += __shfl_xor(dc, off);
-    }
-    if ((t & 63) == 0) {
-        red[0][t >> 6] = acc;
-        red[1][t >> 6] = dc;
-    }
-    __syncthreads();
-    if (t == 0) {
-        double* o = part + ((size_t)bpl * C1_TILES + tile) * 2;
-        o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        for (int off = 32; off > 0; off >>= 1) {
+            acc += __shfl_xor(acc, off);
+            dc += __shfl_xor(dc, off);
+        }
+        if ((t & 63) == 0) {
+            red[0][t >> 6] = acc;
+            red[1][t >> 6] = dc;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double *o = part + ((size_t)x.bpl * C1_TILES + x.tile) * 2;
+            o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+            o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        }
     }
 }
 
@@ -550,9 +590,12 @@ k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp
 // vdif_inout_gpu_thread.cu:578-580): byte (i + 32 ty) = code(t = 2i + 64 ty) |
 // code(t = 2i + 1 + 64 ty) << 4.  Block bytes go to 64 frames of 8192 bytes.
 __global__ void __launch_bounds__(256)
-k_pack(const uint8_t *__restrict__ stage, int bp0, OutDesc od) {
+k_pack(const uint8_t *__restrict__ stage, int bp0, int nb, OutDesc od) {
     __shared__ uint8_t tile[64][65];
-    const int ty0 = blockIdx.x * 64, bpl = blockIdx.y, t = threadIdx.x, bp = bp0 + bpl;
+    // neighbouring ty ranges share 128-byte lines of the code rows: keep them
+    // on one XCD, back to back
+    const int L = xcd_contig((int)blockIdx.x, (NCH / 64) * nb);
+    const int bpl = L >> 8, ty0 = (L & 255) * 64, t = threadIdx.x, bp = bp0 + bpl;
     const uint8_t *src = stage + (size_t)bpl * KHI * NCH + ty0;
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
@@ -757,6 +800,8 @@ int build_tables(sdbe_handle_t *h) {
     };
     size_t total = FFT_TW_N;
     for (int i = 1; i < 6; ++i) total += tw_entries(specs[i]);
+    const size_t c1col_off = total;
+    total += 2 * NCH;
     std::vector<float2> host(total);
     size_t off[6];
     size_t o = FFT_TW_N;
@@ -766,9 +811,20 @@ int build_tables(sdbe_handle_t *h) {
         tw_fill(host.data() + o, specs[i]);
         o += tw_entries(specs[i]);
     }
+    // stage C1 column twiddles e^{-2 pi i c/M} and e^{2 pi i c/L}, c < 16384
+    {
+        const double two_pi = 6.283185307179586476925286766559;
+        for (int c = 0; c < NCH; ++c) {
+            const double am = -two_pi * (double)c / (2.0 * (double)MP);
+            const double al = two_pi * (double)c / (2.0 * (double)P20);
+            host[c1col_off + c] = make_float2((float)cos(am), (float)sin(am));
+            host[c1col_off + NCH + c] = make_float2((float)cos(al), (float)sin(al));
+        }
+    }
     HIP_TRY(h, hipMalloc(&h->tables, total * sizeof(float2)));
     HIP_TRY(h, hipMemcpy(h->tables, host.data(), total * sizeof(float2), hipMemcpyHostToDevice));
     h->tb.fft = h->tables;
+    h->tb.c1col = h->tables + c1col_off;
     Tw2 *dst[6] = {nullptr, &h->tb.n32k, &h->tb.nmp, &h->tb.nm, &h->tb.nl, &h->tb.np};
     for (int i = 1; i < 6; ++i) {
         dst[i]->lo = h->tables + off[i];
@@ -809,11 +865,12 @@ int run_resampler(sdbe_handle_t *h, const SrcDesc &sd, int n_bp, const OutDesc &
     sdsc.group_mode = sd.group_mode;
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
-        LAUNCH(h, KID_A, nb, s, k_stage_a, dim3(NCH / 256, nb), dim3(256), sd, bp0, h->V, h->tb);
+        LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), sd, bp0, nb, h->V, h->tb);
         LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
                (nb % 8) == 0 ? 1 : 0);
-        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(C1_TILES, nb), dim3(256), (const float2 *)h->Zf, h->Bc,
-               h->part, h->tb);
+        const int c1_wgs = (C1_TILES * nb + C1_TILES_PER_WG - 1) / C1_TILES_PER_WG;
+        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(c1_wgs), dim3(256), (const float2 *)h->Zf, h->Bc, h->part, h->tb,
+               nb, C1_TILES_PER_WG);
         LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
         LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(64), (const double2 *)h->ms, nb, bp0, sdsc, h->latch,
                h->thr);
@@ -823,7 +880,8 @@ int run_resampler(sdbe_handle_t *h, const SrcDesc &sd, int n_bp, const OutDesc &
         } else {
             LAUNCH(h, KID_C2, nb, s, k_stage_c2<false>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
                    (const float2 *)h->thr, bp0, od, h->stage, (const int *)h->slot_col, h->tb);
-            LAUNCH(h, KID_PACK, nb, s, k_pack, dim3(NCH / 64, nb), dim3(256), (const uint8_t *)h->stage, bp0, od);
+            LAUNCH(h, KID_PACK, nb, s, k_pack, dim3((NCH / 64) * nb), dim3(256), (const uint8_t *)h->stage, bp0, nb,
+                   od);
         }
         HIP_TRY(h, hipGetLastError());
     }

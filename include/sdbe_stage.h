@@ -94,7 +94,12 @@ typedef struct sdbe_config {
     uint16_t station_id;        /* VDIF w3.stationID (2 ASCII chars, "XX")      */
     uint8_t  bdc_sideband;      /* VDIF w4.bdc_sideband                        */
     uint8_t  rx_sideband;       /* VDIF w4.rx_sideband                         */
-    int32_t reserved[8];
+    int32_t use_graph;          /* sdbe_process_blocks: capture the launch
+                                   sequence once per batch shape as a HIP graph
+                                   and replay it                               */
+    int32_t c1_tiles_per_wg;    /* tuning: stage-C1 tiles per workgroup (1..64, 0 = 8) */
+    int32_t max_batch_blockpols;/* capacity of sdbe_process_blocks (0 = 1024)   */
+    int32_t reserved[5];
 } sdbe_config_t;
 
 /* VDIF output time state for the next frame to be produced; both
@@ -176,6 +181,51 @@ int sdbe_process_spectra(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp,
  * of each block-pol's samples.                                              */
 int sdbe_resample_spectra_f32(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp,
                               float *d_samples, double *d_stats, void *stream);
+
+/* ---- ragged multi-stream batches ---------------------------------------- */
+/* One input stream of a batch (a beam and sideband): n_blocks consecutive
+ * 39-snapshot blocks of both polarisations, already reordered, and where its
+ * VDIF frames go.  Block b of polarisation p reads snapshots 39 b .. 39 b + 38
+ * of d_beng[p] (int8 [..][16384][2 = (im, re)], the layout
+ * sdbe_unpack_group_int8 writes) and writes 64 frames at
+ * d_out + p * out_pol_stride_bytes + 64 b * frame bytes (8192 or 8224).
+ * flip = 1 reverses the channel order first (lower sideband, the spectrum
+ * reversal of software/prototyping/rslib.py:124).  Each stream has its own
+ * threshold policy; REFERENCE streams latch into their own slot
+ * (0 .. SDBE_LATCH_SLOTS - 1, shared with whatever else uses that slot; slot 0
+ * is the one sdbe_process_group uses).  *state is the VDIF time of the
+ * stream's first frame and is advanced by 64 * n_blocks frames.            */
+#define SDBE_LATCH_SLOTS 64
+typedef struct sdbe_stream_desc {
+    const int8_t *d_beng[2];
+    int32_t  n_blocks;
+    int32_t  flip;
+    void    *d_out;
+    int64_t  out_pol_stride_bytes;
+    sdbe_stream_state_t *state;
+    int32_t  thresh_policy;     /* SDBE_THRESH_*                                */
+    float    fixed_threshold;
+    float    fixed_offset;
+    int32_t  latch_slot;
+    uint16_t station_id;
+    uint8_t  bdc_sideband;
+    uint8_t  rx_sideband;
+    int32_t  reserved[3];
+} sdbe_stream_desc_t;
+
+/* Resample, quantise, pack (and, in FRAMES mode, stamp) all blocks of all
+ * streams in one call: the block-pols of every stream are laid end to end and
+ * go through the pipeline in chunks, so streams of uneven length share
+ * launches.  With cfg.use_graph the launch sequence is captured as a HIP graph
+ * the first time a batch of this size is seen and replayed afterwards (the
+ * per-block-pol descriptors are copied in by the graph itself).  No reference
+ * counterpart: the reference processes one stream and one block per
+ * iteration (vdif_inout_gpu_thread.cu:923-1008).                            */
+int sdbe_process_blocks(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int32_t n_streams,
+                        void *stream);
+
+/* Clear REFERENCE latch slot `slot` (asynchronously, on `stream`).          */
+int sdbe_reset_latch(sdbe_handle_t *h, int32_t slot, void *stream);
 
 /* Synchronise the handle's last stream and copy the counters out.          */
 int sdbe_get_stats(sdbe_handle_t *h, sdbe_stats_t *out);

@@ -668,10 +668,11 @@ class GroupOutput:
 
 
 def process_blocks(beng, blocks, policy=THRESH_PER_BLOCK, latch=None, fixed=(1.0, 0.0),
-                   state: StreamState | None = None):
-    """Resample, threshold, quantise and pack the given blocks of one group
-    (both pols).  ``latch`` is a one-element list holding the REFERENCE-policy
-    (th, off) across calls."""
+                   state: StreamState | None = None, flip=False):
+    """Resample, threshold, quantise and pack the given blocks of one group or
+    stream (both pols).  ``latch`` is a one-element list holding the
+    REFERENCE-policy (th, off) across calls.  ``flip`` reverses the channel
+    order of every spectrum first (lower sideband, rslib.py:124)."""
     if latch is None:
         latch = [None]
     samples, stats, ths, codes = {}, {}, {}, {}
@@ -679,7 +680,8 @@ def process_blocks(beng, blocks, policy=THRESH_PER_BLOCK, latch=None, fixed=(1.0
     payload = np.zeros((2, nb * OUT_FRAMES_PER_BLOCK, OUT_PAYLOAD_BYTES // 4), dtype=np.uint32)
     for bi, blk in enumerate(blocks):
         for pol in (0, 1):
-            x = resample_block(block_spectra(beng[pol], blk))
+            X = block_spectra(beng[pol], blk)
+            x = resample_block(flip_sideband(X) if flip else X)
             samples[(blk, pol)] = x
             stats[(blk, pol)] = block_stats(x)
         for pol in (0, 1):

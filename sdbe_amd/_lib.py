@@ -46,7 +46,10 @@ class Config(C.Structure):
         ("station_id", C.c_uint16),
         ("bdc_sideband", C.c_uint8),
         ("rx_sideband", C.c_uint8),
-        ("reserved", C.c_int32 * 8),
+        ("use_graph", C.c_int32),
+        ("c1_tiles_per_wg", C.c_int32),
+        ("max_batch_blockpols", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -57,6 +60,28 @@ class StreamState(C.Structure):
         ("ref_epoch", C.c_uint32),
         ("reserved", C.c_uint32),
         ("psn", C.c_uint64),
+    ]
+
+
+LATCH_SLOTS = 64
+
+
+class StreamDesc(C.Structure):
+    _fields_ = [
+        ("d_beng", C.c_void_p * 2),
+        ("n_blocks", C.c_int32),
+        ("flip", C.c_int32),
+        ("d_out", C.c_void_p),
+        ("out_pol_stride_bytes", C.c_int64),
+        ("state", C.POINTER(StreamState)),
+        ("thresh_policy", C.c_int32),
+        ("fixed_threshold", C.c_float),
+        ("fixed_offset", C.c_float),
+        ("latch_slot", C.c_int32),
+        ("station_id", C.c_uint16),
+        ("bdc_sideband", C.c_uint8),
+        ("rx_sideband", C.c_uint8),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -103,6 +128,8 @@ PROTOTYPES = [
     ("sdbe_process_spectra", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("sdbe_resample_spectra_f32", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("sdbe_process_blocks", C.c_int, [C.c_void_p, C.POINTER(StreamDesc), C.c_int32, C.c_void_p]),
+    ("sdbe_reset_latch", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("sdbe_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("sdbe_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("sdbe_profile_read", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int32, C.POINTER(C.c_int32)]),

@@ -45,7 +45,7 @@ constexpr int MAP_NONE = 0x7FFFFFFF;
 // (c = a + 39 (16 u + j)), 14 tiles per class, 39 classes.
 constexpr int C1_CLASS_TILES = 14;
 constexpr int C1_TILES = NBLK * C1_CLASS_TILES;   // 546
-constexpr int C1_TILES_PER_WG = 8;                 // consecutive tiles run by one stage C1 workgroup
+constexpr int C1_TILES_PER_WG = 8;                 // default consecutive tiles per stage C1 workgroup
 constexpr int BC_ROW = C1_TILES * 32;            // 17472 slots per row of the first-pass output
 constexpr int OUT_FRAME_PAYLOAD = 8192;
 constexpr int OUT_FRAMES_BLOCK = 64;
@@ -159,41 +159,61 @@ __device__ __forceinline__ int xcd_contig(int lin, int total) {
     return x * q + (x < r ? x : r) + slot;
 }
 
-struct SrcDesc {
-    const int8_t *i8;        // int8 [..][16384][2 = (im, re)] rows
-    int group_mode;          // 1: rows [2][4992] of a group, bp = 2 blk + pol
+// One block-pol of a call: where its spectra come from, where its frames go,
+// how it is thresholded and how its frames are stamped.  Built on the host for
+// every call and copied to the device ahead of the kernels.
+struct BpDesc {
+    const int8_t *rows;      // 39 rows of int8 [16384][2 = (im, re)]
+    uint8_t *out;            // first of its 64 frames (header included in FRAMES mode) or float samples
+    uint64_t psn;            // VDIF time of its first frame
+    uint32_t secs, df;
+    uint32_t w1hi;           // ref_epoch << 24
+    uint32_t w3;             // stationID | (bps = 1) << 26
+    uint32_t w4;             // pol | bdc << 1 | rx << 2 | (edv5 = 2) << 24
+    int16_t flip, pol, policy, slot;
+    float fixed_th, fixed_off;
+    uint32_t pad;
 };
+static_assert(sizeof(BpDesc) == 64, "BpDesc layout");
+
+// Pointers read from a descriptor are generic to the compiler, which would
+// then emit flat memory instructions (counted on both the vector-memory and
+// LDS counters, so that every load waits for the previous ones).  These casts
+// state that they address global memory.
+#define GAS __attribute__((address_space(1)))
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ GAS T *gptr(T *p) {
+    return (GAS T *)p;
+}
 
 // Stage A: thread per channel k of a block-pol.  For each snapshot s the
 // packed C2R input Z_s[k] = (X[k] + conj(X[N2-k])) + i e^{2 pi i k/32768}
 // (X[k] - conj(X[N2-k])) (DC imaginary part and the zero Nyquist bin dropped,
 // as numpy irfft), then V[k1][k] = sum_s Z_s[k] e^{-2 pi i s k1/39}.
+// A flipped (lower-sideband) block-pol reads channel 16383 - k in place of k
+// (the Nyquist slot stays zero).
 // Workgroups are paired (w, 63 - w) on one XCD back to back, so the mirror
 // channels N2 - k a workgroup reads are the ones its partner reads as k and
 // are served from that XCD's L2.
 __global__ void __launch_bounds__(256, 2)
-k_stage_a(SrcDesc sd, int bp0, int nb, float2 *__restrict__ V, Tables tb) {
+k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables tb) {
     const int L = xcd_contig((int)blockIdx.x, (NCH / 256) * nb);
     const int bpl = L >> 6, idx = L & 63;
     const int w = (idx & 1) ? 63 - (idx >> 1) : (idx >> 1);
     const int k = w * 256 + threadIdx.x;
-    const int bp = bp0 + bpl;
-    size_t row0;
-    if (sd.group_mode) {
-        const int blk = bp >> 1, pol = bp & 1;
-        row0 = (size_t)pol * NSNAP_GROUP + (size_t)NBLK * blk;
-    } else {
-        row0 = (size_t)bp * NBLK;
-    }
-    const char2 *src = reinterpret_cast<const char2 *>(sd.i8) + row0 * NCH;
+    const GAS uint16_t *src = gptr(reinterpret_cast<const uint16_t *>(bpd[bpl].rows));
+    const int flip = bpd[bpl].flip;
     const int km = (NCH - k) & (NCH - 1);      // k = 0 pairs with the zero Nyquist bin
+    const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
     const float kz = k == 0 ? 0.f : 1.f;
     const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
     float2 x[NBLK];
 #pragma unroll
     for (int s = 0; s < NBLK; ++s) {
-        const char2 va = src[(size_t)s * NCH + k];
-        const char2 vb = src[(size_t)s * NCH + km];
+        const uint16_t ua = src[(size_t)s * NCH + ka], ub = src[(size_t)s * NCH + kb];
+        const char2 va = make_char2((int8_t)(ua & 0xFF), (int8_t)(ua >> 8));   // (im, re)
+        const char2 vb = make_char2((int8_t)(ub & 0xFF), (int8_t)(ub >> 8));
         const float2 a = make_float2((float)va.y, kz * (float)va.x);     // DC imaginary part dropped
         const float2 b = make_float2(kz * (float)vb.y, kz * (float)vb.x);  // Nyquist bin is zero
         const float2 sum = cadd(a, cconj(b));
@@ -431,12 +451,6 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
 // 32768 * 2^21 = 2^36 times numpy's; Y here carries a factor 32768.
 //   sum x^2 = W / (32768^2 * L),  mean = Re Y[0] / (32768 * L),
 //   sigma = sqrt((sum x^2 - L mean^2) / (L - 1))
-struct StatsDesc {
-    int policy;
-    float fixed_th, fixed_off;
-    int group_mode;
-};
-
 __global__ void __launch_bounds__(256)
 k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, double *__restrict__ stats_out) {
     __shared__ double red[2][256];
@@ -471,39 +485,49 @@ k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, doub
     }
 }
 
-// Threshold policy over a chunk, in block order (one thread: the REFERENCE
-// latch is sequential by definition).
-__global__ void k_thresh(const double2 *__restrict__ ms, int nbpl, int bp0, StatsDesc sdsc,
-                         float *__restrict__ latch, float2 *__restrict__ thr) {
+// Threshold policy over a chunk, in block-pol order (one thread: the REFERENCE
+// latch is sequential by definition).  REFERENCE: the first polarisation-0
+// block-pol seen while the slot's threshold is 0 latches its sigma and mean,
+// which then serve both polarisations (vdif_inout_gpu_thread.cu:954-1005).
+// The chunk's statistics and policy fields are staged in LDS by all threads
+// first, so that the sequential pass does not wait on one load per block-pol.
+constexpr int THRESH_MAX_CHUNK = 256;
+__global__ void __launch_bounds__(256) k_thresh(const double2 *__restrict__ ms, int nbpl,
+                                                const BpDesc *__restrict__ bpd, float *__restrict__ latch,
+                                                float2 *__restrict__ thr) {
+    __shared__ float2 sms[THRESH_MAX_CHUNK];      // (sigma, mean) as float
+    __shared__ float2 sfix[THRESH_MAX_CHUNK];
+    __shared__ int sinfo[THRESH_MAX_CHUNK];       // policy | pol << 8 | slot << 16
+    for (int b = threadIdx.x; b < nbpl; b += blockDim.x) {
+        const BpDesc &d = bpd[b];
+        sms[b] = make_float2((float)ms[b].y, (float)ms[b].x);
+        sfix[b] = make_float2(d.fixed_th, d.fixed_off);
+        sinfo[b] = (int)d.policy | ((int)d.pol << 8) | ((int)d.slot << 16);
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     for (int b = 0; b < nbpl; ++b) {
-        float th, off;
-        const int pol = sdsc.group_mode ? ((bp0 + b) & 1) : 0;
-        if (sdsc.policy == SDBE_THRESH_PER_BLOCK) {
-            th = (float)ms[b].y;
-            off = (float)ms[b].x;
-        } else if (sdsc.policy == SDBE_THRESH_REFERENCE) {
-            if (latch[0] == 0.0f && pol == 0) {
-                latch[0] = (float)ms[b].y;
-                latch[1] = (float)ms[b].x;
+        const int info = sinfo[b], policy = info & 0xFF, pol = (info >> 8) & 0xFF, slot = info >> 16;
+        float2 to;
+        if (policy == SDBE_THRESH_PER_BLOCK) {
+            to = sms[b];
+        } else if (policy == SDBE_THRESH_REFERENCE) {
+            float *l = latch + 2 * slot;
+            if (l[0] == 0.0f && pol == 0) {
+                l[0] = sms[b].x;
+                l[1] = sms[b].y;
             }
-            th = latch[0];
-            off = latch[1];
+            to = make_float2(l[0], l[1]);
         } else {
-            th = sdsc.fixed_th;
-            off = sdsc.fixed_off;
+            to = sfix[b];
         }
-        thr[b] = make_float2(th, off);
+        thr[b] = to;
     }
 }
 
 struct OutDesc {
-    uint8_t *base;
-    int64_t pol_stride;
-    int32_t frame_bytes;
-    int32_t hdr_bytes;
-    int32_t group_mode;
-    float *f32;          // resample-only output [n_bp][2^21] or null
+    int32_t frame_bytes;     // 8192 (PAYLOAD) or 8224 (FRAMES)
+    int32_t hdr_bytes;       // 0 or 32
 };
 
 // Stage C2: workgroup = (tx, block-pol).
@@ -513,13 +537,13 @@ struct OutDesc {
 // the 4-bit pair code (re | im << 2) of t goes to stage[bp][tx][ty].
 template <bool F32>
 __global__ void __launch_bounds__(1024)
-k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp0, OutDesc od,
+k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, const BpDesc *__restrict__ bpd,
            uint8_t *__restrict__ stage, const int *__restrict__ slot_col, Tables tb) {
     __shared__ float lre[LDS_PAD_N];
     __shared__ float lim[LDS_PAD_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, tb.fft);
-    const int tx = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x, bp = bp0 + bpl;
+    const int tx = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
     const float scale = 1.4551915228366852e-11f;   // 2^-36, exact
     // the row arrives in stage C1's tile-major slot order: scatter it into LDS in
     // column order (slot -> column table, -1 for slots without a column)
@@ -568,9 +592,12 @@ k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp
     for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
     fft16k<+1>(v, lre, lim, &ftw);
     if (F32) {
-        float2 *dst = reinterpret_cast<float2 *>(od.f32) + (size_t)bp * P20 + tx + KHI * t;
+        GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * t;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) dst[(size_t)KHI * 1024 * q] = cscale(v[q], scale);
+        for (int q = 0; q < 16; ++q) {
+            const float2 y = cscale(v[q], scale);
+            dst[(size_t)KHI * 1024 * q] = (uint64_t)__float_as_uint(y.x) | ((uint64_t)__float_as_uint(y.y) << 32);
+        }
     } else {
         const float2 to = thr[bpl];
         uint8_t *dst = stage + ((size_t)bpl * KHI + tx) * NCH + t;
@@ -590,12 +617,12 @@ k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, int bp
 // vdif_inout_gpu_thread.cu:578-580): byte (i + 32 ty) = code(t = 2i + 64 ty) |
 // code(t = 2i + 1 + 64 ty) << 4.  Block bytes go to 64 frames of 8192 bytes.
 __global__ void __launch_bounds__(256)
-k_pack(const uint8_t *__restrict__ stage, int bp0, int nb, OutDesc od) {
+k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb, OutDesc od) {
     __shared__ uint8_t tile[64][65];
     // neighbouring ty ranges share 128-byte lines of the code rows: keep them
     // on one XCD, back to back
     const int L = xcd_contig((int)blockIdx.x, (NCH / 64) * nb);
-    const int bpl = L >> 8, ty0 = (L & 255) * 64, t = threadIdx.x, bp = bp0 + bpl;
+    const int bpl = L >> 8, ty0 = (L & 255) * 64, t = threadIdx.x;
     const uint8_t *src = stage + (size_t)bpl * KHI * NCH + ty0;
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
@@ -603,17 +630,8 @@ k_pack(const uint8_t *__restrict__ stage, int bp0, int nb, OutDesc od) {
         tile[ty][tx] = src[(size_t)tx * NCH + ty];
     }
     __syncthreads();
-    int pol, blk;
-    if (od.group_mode) {
-        pol = bp & 1;
-        blk = bp >> 1;
-    } else {
-        pol = 0;
-        blk = bp;
-    }
     // 64 ty x 32 bytes = 2048 contiguous stream bytes starting at 32 * ty0
-    uint8_t *ob = od.base + (int64_t)pol * od.pol_stride + (int64_t)blk * OUT_FRAMES_BLOCK * od.frame_bytes +
-                  od.hdr_bytes;
+    GAS uint8_t *ob = gptr(bpd[bpl].out) + od.hdr_bytes;
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
         const int e = it * 256 + t;            // 4-byte word index within the 2048 bytes
@@ -626,42 +644,27 @@ k_pack(const uint8_t *__restrict__ stage, int bp0, int nb, OutDesc od) {
         }
         const int qb = 32 * (ty0 + ty) + i0;    // stream byte
         const int f = qb >> 13, off = qb & (OUT_FRAME_PAYLOAD - 1);
-        *reinterpret_cast<uint32_t *>(ob + (int64_t)f * od.frame_bytes + off) = wv;
+        *reinterpret_cast<GAS uint32_t *>(ob + (int64_t)f * od.frame_bytes + off) = wv;
     }
 }
 
-struct HdrDesc {
-    uint32_t secs, df, ref_epoch;
-    uint64_t psn;
-    uint32_t station, bdc, rx;
-    int n_frames_per_pol;
-    int n_pols;
-};
-
 // VDIF output headers (vdif_out_databuf.h:19-52) stamped as the output thread
 // does (vdif_out_net_thread.c:294-317, defaults vdif_out_databuf.c:195-227):
-// frame n of the call has df = (df0 + n) mod 125000, secs = secs0 + carry,
-// psn = psn0 + n, threadID 0, bps = 1, df_len = 8224/8, edv5 = 2.
-__global__ void __launch_bounds__(256) k_headers(OutDesc od, HdrDesc hd) {
+// frame n of a block-pol whose first frame is (secs, df, psn) has
+// df = (df + n) mod 125000, secs + carry, psn + n; threadID 0, bps = 1,
+// df_len = 8224 / 8, edv5 = 2 (words 3 and 4 come prebuilt per block-pol).
+__global__ void __launch_bounds__(256) k_headers(const BpDesc *__restrict__ bpd, int n_bp, OutDesc od) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= hd.n_frames_per_pol * hd.n_pols) return;
-    const int pol = i / hd.n_frames_per_pol, n = i - pol * hd.n_frames_per_pol;
-    const uint64_t dft = (uint64_t)hd.df + (uint64_t)n;
+    if (i >= n_bp * OUT_FRAMES_BLOCK) return;
+    const int b = i / OUT_FRAMES_BLOCK, n = i - b * OUT_FRAMES_BLOCK;
+    const BpDesc &d = bpd[b];
+    const uint64_t dft = (uint64_t)d.df + (uint64_t)n;
     const uint32_t df = (uint32_t)(dft % 125000u);
-    const uint32_t secs = hd.secs + (uint32_t)(dft / 125000u);
-    const uint64_t psn = hd.psn + (uint64_t)n;
-    uint32_t w[8];
-    w[0] = secs & 0x3FFFFFFFu;
-    w[1] = (df & 0xFFFFFFu) | ((hd.ref_epoch & 0x3Fu) << 24);
-    w[2] = (uint32_t)((OUT_FRAME_PAYLOAD + 32) / 8);
-    w[3] = (hd.station & 0xFFFFu) | (1u << 26);
-    w[4] = ((uint32_t)pol & 1u) | ((hd.bdc & 1u) << 1) | ((hd.rx & 1u) << 2) | (2u << 24);
-    w[5] = 0u;
-    w[6] = (uint32_t)(psn & 0xFFFFFFFFu);
-    w[7] = (uint32_t)(psn >> 32);
-    uint4 *dst = reinterpret_cast<uint4 *>(od.base + (int64_t)pol * od.pol_stride + (int64_t)n * od.frame_bytes);
-    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    const uint32_t secs = d.secs + (uint32_t)(dft / 125000u);
+    const uint64_t psn = d.psn + (uint64_t)n;
+    GAS u32x4 *dst = reinterpret_cast<GAS u32x4 *>(gptr(d.out) + (int64_t)n * od.frame_bytes);
+    dst[0] = u32x4{secs & 0x3FFFFFFFu, (df & 0xFFFFFFu) | d.w1hi, (uint32_t)((OUT_FRAME_PAYLOAD + 32) / 8), d.w3};
+    dst[1] = u32x4{d.w4, 0u, (uint32_t)(psn & 0xFFFFFFFFu), (uint32_t)(psn >> 32)};
 }
 
 }  // namespace
@@ -702,6 +705,24 @@ struct sdbe_handle {
     double2 *ms = nullptr;
     uint8_t *stage = nullptr;
     int *slot_col = nullptr;         // stage C1 output slot -> column (or -1)
+    // per-block-pol descriptors: a ring of host (pinned) / device tables, each
+    // reused only after the call that last used it has finished with it
+    static constexpr int RING = 4;
+    int cap_bp = 0;
+    BpDesc *h_bpd[RING] = {};
+    BpDesc *d_bpd[RING] = {};
+    hipEvent_t bpd_done[RING] = {};
+    int bpd_used[RING] = {};
+    int ring_i = 0;
+    // graph replay of sdbe_process_blocks (own descriptor table)
+    int capturing = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_nbp = 0;
+    BpDesc *g_h = nullptr, *g_d = nullptr;
+    hipEvent_t g_done = nullptr;
+    int g_used = 0;
+    int c1_tpw = 8;
     // pinned host mirrors
     int *h_cnt = nullptr;
     float *h_latch = nullptr;
@@ -727,7 +748,7 @@ int hip_fail(sdbe_handle_t *h, hipError_t e) {
     } while (0)
 
 ProfRec *prof_begin(sdbe_handle_t *h, int kid, double units, hipStream_t s) {
-    if (!h->prof) return nullptr;
+    if (!h->prof || h->capturing) return nullptr;
     hipEvent_t ev[2];
     for (int i = 0; i < 2; ++i) {
         if (!h->pool.empty()) {
@@ -856,36 +877,95 @@ int build_slot_table(sdbe_handle_t *h) {
 }
 
 // Run stages A..C2 over block-pols [0, n_bp) in chunks.
-int run_resampler(sdbe_handle_t *h, const SrcDesc &sd, int n_bp, const OutDesc &od,
+// Run stages A..C2 (+ pack) over the n_bp block-pols described by the device
+// table d_bpd, in chunks.  f32: write float samples instead of codes.
+int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32,
                   double *d_stats, hipStream_t s) {
-    StatsDesc sdsc;
-    sdsc.policy = h->cfg.thresh_policy;
-    sdsc.fixed_th = h->cfg.fixed_threshold;
-    sdsc.fixed_off = h->cfg.fixed_offset;
-    sdsc.group_mode = sd.group_mode;
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
-        LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), sd, bp0, nb, h->V, h->tb);
+        const BpDesc *bpd = d_bpd + bp0;
+        LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb);
         LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
                (nb % 8) == 0 ? 1 : 0);
-        const int c1_wgs = (C1_TILES * nb + C1_TILES_PER_WG - 1) / C1_TILES_PER_WG;
+        const int c1_wgs = (C1_TILES * nb + h->c1_tpw - 1) / h->c1_tpw;
         LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(c1_wgs), dim3(256), (const float2 *)h->Zf, h->Bc, h->part, h->tb,
-               nb, C1_TILES_PER_WG);
+               nb, h->c1_tpw);
         LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
-        LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(64), (const double2 *)h->ms, nb, bp0, sdsc, h->latch,
-               h->thr);
-        if (od.f32) {
+        LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(256), (const double2 *)h->ms, nb, bpd, h->latch, h->thr);
+        if (f32) {
             LAUNCH(h, KID_C2, nb, s, k_stage_c2<true>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
-                   (const float2 *)h->thr, bp0, od, h->stage, (const int *)h->slot_col, h->tb);
+                   (const float2 *)h->thr, bpd, h->stage, (const int *)h->slot_col, h->tb);
         } else {
             LAUNCH(h, KID_C2, nb, s, k_stage_c2<false>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
-                   (const float2 *)h->thr, bp0, od, h->stage, (const int *)h->slot_col, h->tb);
-            LAUNCH(h, KID_PACK, nb, s, k_pack, dim3((NCH / 64) * nb), dim3(256), (const uint8_t *)h->stage, bp0, nb,
+                   (const float2 *)h->thr, bpd, h->stage, (const int *)h->slot_col, h->tb);
+            LAUNCH(h, KID_PACK, nb, s, k_pack, dim3((NCH / 64) * nb), dim3(256), (const uint8_t *)h->stage, bpd, nb,
                    od);
         }
         HIP_TRY(h, hipGetLastError());
     }
+    if (!f32 && od.hdr_bytes) {
+        LAUNCH(h, KID_HEADERS, n_bp, s, k_headers, dim3((n_bp * OUT_FRAMES_BLOCK + 255) / 256), dim3(256), d_bpd,
+               n_bp, od);
+        HIP_TRY(h, hipGetLastError());
+    }
     return SDBE_OK;
+}
+
+OutDesc out_desc(const sdbe_handle_t *h) {
+    OutDesc od;
+    od.hdr_bytes = h->cfg.out_mode == SDBE_OUT_FRAMES ? 32 : 0;
+    od.frame_bytes = OUT_FRAME_PAYLOAD + od.hdr_bytes;
+    return od;
+}
+
+// Fill one descriptor.  t0 is the VDIF time of the block-pol's first frame.
+void set_bp(BpDesc &d, const int8_t *rows, uint8_t *out, const sdbe_stream_state_t &t0, int pol, int flip,
+            int policy, int slot, float fth, float foff, uint16_t station, uint8_t bdc, uint8_t rx) {
+    memset(&d, 0, sizeof(d));
+    d.rows = rows;
+    d.out = out;
+    d.secs = t0.secs;
+    d.df = t0.df;
+    d.psn = t0.psn;
+    d.w1hi = (t0.ref_epoch & 0x3Fu) << 24;
+    d.w3 = (uint32_t)station | (1u << 26);
+    d.w4 = ((uint32_t)pol & 1u) | ((uint32_t)(bdc & 1u) << 1) | ((uint32_t)(rx & 1u) << 2) | (2u << 24);
+    d.flip = (int16_t)(flip ? 1 : 0);
+    d.pol = (int16_t)pol;
+    d.policy = (int16_t)policy;
+    d.slot = (int16_t)slot;
+    d.fixed_th = fth;
+    d.fixed_off = foff;
+}
+
+// Next descriptor table of the ring (waits until the call that last used it
+// has finished on the device).
+int ring_acquire(sdbe_handle_t *h, int &slot) {
+    slot = h->ring_i;
+    h->ring_i = (h->ring_i + 1) % sdbe_handle::RING;
+    if (h->bpd_used[slot]) HIP_TRY(h, hipEventSynchronize(h->bpd_done[slot]));
+    h->bpd_used[slot] = 0;
+    return SDBE_OK;
+}
+
+int ring_upload(sdbe_handle_t *h, int slot, int n_bp, hipStream_t s) {
+    HIP_TRY(h, hipMemcpyAsync(h->d_bpd[slot], h->h_bpd[slot], (size_t)n_bp * sizeof(BpDesc),
+                              hipMemcpyHostToDevice, s));
+    return SDBE_OK;
+}
+
+int ring_release(sdbe_handle_t *h, int slot, hipStream_t s) {
+    HIP_TRY(h, hipEventRecord(h->bpd_done[slot], s));
+    h->bpd_used[slot] = 1;
+    return SDBE_OK;
+}
+
+void advance_state(sdbe_stream_state_t &st, int64_t n_frames);
+
+sdbe_stream_state_t state_plus(const sdbe_stream_state_t &st, int64_t n_frames) {
+    sdbe_stream_state_t t = st;
+    advance_state(t, n_frames);
+    return t;
 }
 
 int unpack_to_rows(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero,
@@ -968,6 +1048,16 @@ void sdbe_destroy(sdbe_handle_t *h) {
     (void)hipFree(h->ms);
     (void)hipHostFree(h->h_cnt);
     (void)hipHostFree(h->h_latch);
+    for (int i = 0; i < sdbe_handle::RING; ++i) {
+        (void)hipFree(h->d_bpd[i]);
+        (void)hipHostFree(h->h_bpd[i]);
+        if (h->bpd_done[i]) (void)hipEventDestroy(h->bpd_done[i]);
+    }
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    (void)hipFree(h->g_d);
+    (void)hipHostFree(h->g_h);
+    if (h->g_done) (void)hipEventDestroy(h->g_done);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     for (auto &r : h->recs) {
         (void)hipEventDestroy(r.e0);
         (void)hipEventDestroy(r.e1);
@@ -985,7 +1075,8 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     if (cfg.thresh_policy < SDBE_THRESH_REFERENCE || cfg.thresh_policy > SDBE_THRESH_FIXED) return SDBE_ERR_ARG;
     if (cfg.out_mode != SDBE_OUT_PAYLOAD && cfg.out_mode != SDBE_OUT_FRAMES) return SDBE_ERR_ARG;
     if (cfg.chunk_blockpols < 2 || cfg.chunk_blockpols > 256 || (cfg.chunk_blockpols & 1)) return SDBE_ERR_ARG;
-    if (cfg.max_spectra_blockpols < 1) return SDBE_ERR_ARG;
+    if (cfg.max_spectra_blockpols < 1 || cfg.max_batch_blockpols < 0) return SDBE_ERR_ARG;
+    if (cfg.c1_tiles_per_wg < 0 || cfg.c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SDBE_ERR_NODEV;
     sdbe_handle_t *h = new (std::nothrow) sdbe_handle();
@@ -993,6 +1084,13 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     h->device = device;
     h->cfg = cfg;
     h->chunk = cfg.chunk_blockpols;
+    h->c1_tpw = cfg.c1_tiles_per_wg ? cfg.c1_tiles_per_wg : C1_TILES_PER_WG;
+    h->cap_bp = 2 * SDBE_BLOCKS_PER_GROUP;
+    if (cfg.max_spectra_blockpols > h->cap_bp) h->cap_bp = cfg.max_spectra_blockpols;
+    {
+        const int mb = cfg.max_batch_blockpols ? cfg.max_batch_blockpols : 1024;
+        if (mb > h->cap_bp) h->cap_bp = mb;
+    }
     h->st.psn = 1;
     int rc = SDBE_OK;
     hipError_t e = hipSetDevice(device);
@@ -1012,7 +1110,9 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->beng, 2 * (size_t)NSNAP_GROUP * NCH * 2);
     ALLOC(h->bitmap, NFR * 256);
     ALLOC(h->cnt, CNT_N * sizeof(int));
-    ALLOC(h->latch, 2 * sizeof(float));
+    ALLOC(h->latch, 2 * SDBE_LATCH_SLOTS * sizeof(float));
+    for (int i = 0; i < sdbe_handle::RING; ++i) ALLOC(h->d_bpd[i], (size_t)h->cap_bp * sizeof(BpDesc));
+    ALLOC(h->g_d, (size_t)h->cap_bp * sizeof(BpDesc));
     ALLOC(h->V, C * NBLK * NCH * sizeof(float2));
     ALLOC(h->Zf, C * MP * sizeof(float2));
     ALLOC(h->Bc, C * KHI * BC_ROW * sizeof(float2));
@@ -1027,6 +1127,19 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
         rc = SDBE_ERR_NOMEM;
         goto fail;
     }
+    for (int i = 0; i < sdbe_handle::RING; ++i) {
+        if (hipHostMalloc((void **)&h->h_bpd[i], (size_t)h->cap_bp * sizeof(BpDesc), hipHostMallocDefault) !=
+                hipSuccess ||
+            hipEventCreateWithFlags(&h->bpd_done[i], hipEventDisableTiming) != hipSuccess) {
+            rc = SDBE_ERR_NOMEM;
+            goto fail;
+        }
+    }
+    if (hipHostMalloc((void **)&h->g_h, (size_t)h->cap_bp * sizeof(BpDesc), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&h->g_done, hipEventDisableTiming) != hipSuccess) {
+        rc = SDBE_ERR_NOMEM;
+        goto fail;
+    }
     memset(h->h_cnt, 0, CNT_N * sizeof(int));
     memset(h->h_latch, 0, 2 * sizeof(float));
     rc = build_tables(h);
@@ -1034,7 +1147,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     rc = build_slot_table(h);
     if (rc != SDBE_OK) goto fail;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, h->cnt, CNT_N, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, (int *)h->latch, 2, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, (int *)h->latch, 2 * SDBE_LATCH_SLOTS, 0);
     if (hipDeviceSynchronize() != hipSuccess) {
         rc = SDBE_ERR_HIP;
         goto fail;
@@ -1108,75 +1221,66 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
                        int64_t out_pol_stride_bytes, uint8_t *d_bitmap, void *stream) {
     if (!h || !d_out || (n_pkts > 0 && !d_pkts)) return SDBE_ERR_ARG;
     if (n_pkts < 0 || n_pkts > MAP_N) return SDBE_ERR_CAPACITY;
-    const int frame_bytes = h->cfg.out_mode == SDBE_OUT_FRAMES ? OUT_FRAME_PAYLOAD + 32 : OUT_FRAME_PAYLOAD;
-    if (out_pol_stride_bytes < (int64_t)SDBE_OUT_FRAMES_PER_GROUP * frame_bytes) return SDBE_ERR_ARG;
+    const OutDesc od = out_desc(h);
+    if (out_pol_stride_bytes < (int64_t)SDBE_OUT_FRAMES_PER_GROUP * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
     int rc = unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;
-    SrcDesc sd = {h->beng, 1};
-    OutDesc od;
-    od.base = (uint8_t *)d_out;
-    od.pol_stride = out_pol_stride_bytes;
-    od.frame_bytes = frame_bytes;
-    od.hdr_bytes = h->cfg.out_mode == SDBE_OUT_FRAMES ? 32 : 0;
-    od.group_mode = 1;
-    od.f32 = nullptr;
-    rc = run_resampler(h, sd, 2 * SDBE_BLOCKS_PER_GROUP, od, nullptr, s);
-    if (rc) return rc;
-    if (h->cfg.out_mode == SDBE_OUT_FRAMES) {
-        HdrDesc hd;
-        hd.secs = h->st.secs;
-        hd.df = h->st.df;
-        hd.ref_epoch = h->st.ref_epoch;
-        hd.psn = h->st.psn;
-        hd.station = h->cfg.station_id;
-        hd.bdc = h->cfg.bdc_sideband;
-        hd.rx = h->cfg.rx_sideband;
-        hd.n_frames_per_pol = SDBE_OUT_FRAMES_PER_GROUP;
-        hd.n_pols = 2;
-        LAUNCH(h, KID_HEADERS, 2 * SDBE_BLOCKS_PER_GROUP, s, k_headers,
-               dim3((2 * SDBE_OUT_FRAMES_PER_GROUP + 255) / 256), dim3(256), od, hd);
-        HIP_TRY(h, hipGetLastError());
+    int slot;
+    if ((rc = ring_acquire(h, slot))) return rc;
+    const int n_bp = 2 * SDBE_BLOCKS_PER_GROUP;
+    BpDesc *hb = h->h_bpd[slot];
+    const sdbe_config_t &c = h->cfg;
+    for (int bp = 0; bp < n_bp; ++bp) {
+        const int blk = bp >> 1, pol = bp & 1;
+        const int8_t *rows = h->beng + ((size_t)pol * NSNAP_GROUP + (size_t)NBLK * blk) * NCH * 2;
+        uint8_t *out = (uint8_t *)d_out + (int64_t)pol * out_pol_stride_bytes +
+                       (int64_t)blk * OUT_FRAMES_BLOCK * od.frame_bytes;
+        set_bp(hb[bp], rows, out, state_plus(h->st, (int64_t)blk * OUT_FRAMES_BLOCK), pol, 0, c.thresh_policy, 0,
+               c.fixed_threshold, c.fixed_offset, c.station_id, c.bdc_sideband, c.rx_sideband);
     }
+    if ((rc = ring_upload(h, slot, n_bp, s))) return rc;
+    rc = run_resampler(h, h->d_bpd[slot], n_bp, od, false, nullptr, s);
+    if (rc) return rc;
+    if ((rc = ring_release(h, slot, s))) return rc;
     advance_state(h->st, SDBE_OUT_FRAMES_PER_GROUP);
     h->n_packets_last = n_pkts;
-    h->n_bp_last = 2 * SDBE_BLOCKS_PER_GROUP;
+    h->n_bp_last = n_bp;
     h->have_group = 1;
     return finish_call(h, s);
 }
+
+namespace {
+// Descriptors of n_bp independent block-pols laid out back to back
+// (sdbe_process_spectra / sdbe_resample_spectra_f32).
+int spectra_call(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp, uint8_t *out, int64_t out_bp_stride,
+                 bool f32, double *d_stats, hipStream_t s) {
+    int slot, rc;
+    if ((rc = ring_acquire(h, slot))) return rc;
+    BpDesc *hb = h->h_bpd[slot];
+    const sdbe_config_t &c = h->cfg;
+    for (int bp = 0; bp < n_bp; ++bp)
+        set_bp(hb[bp], d_beng + (size_t)bp * NBLK * NCH * 2, out + (int64_t)bp * out_bp_stride,
+               state_plus(h->st, (int64_t)bp * OUT_FRAMES_BLOCK), 0, 0, c.thresh_policy, 0, c.fixed_threshold,
+               c.fixed_offset, c.station_id, c.bdc_sideband, c.rx_sideband);
+    if ((rc = ring_upload(h, slot, n_bp, s))) return rc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h->cnt, CNT_N, 0);
+    rc = run_resampler(h, h->d_bpd[slot], n_bp, out_desc(h), f32, d_stats, s);
+    if (rc) return rc;
+    return ring_release(h, slot, s);
+}
+}  // namespace
 
 int sdbe_process_spectra(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp, void *d_out, void *stream) {
     if (!h || !d_beng || !d_out || n_bp <= 0) return SDBE_ERR_ARG;
     if (n_bp > h->cfg.max_spectra_blockpols) return SDBE_ERR_CAPACITY;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    const int frame_bytes = h->cfg.out_mode == SDBE_OUT_FRAMES ? OUT_FRAME_PAYLOAD + 32 : OUT_FRAME_PAYLOAD;
-    SrcDesc sd = {d_beng, 0};
-    OutDesc od;
-    od.base = (uint8_t *)d_out;
-    od.pol_stride = 0;
-    od.frame_bytes = frame_bytes;
-    od.hdr_bytes = h->cfg.out_mode == SDBE_OUT_FRAMES ? 32 : 0;
-    od.group_mode = 0;
-    od.f32 = nullptr;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h->cnt, CNT_N, 0);
-    int rc = run_resampler(h, sd, n_bp, od, nullptr, s);
+    const OutDesc od = out_desc(h);
+    int rc = spectra_call(h, d_beng, n_bp, (uint8_t *)d_out, (int64_t)OUT_FRAMES_BLOCK * od.frame_bytes, false,
+                          nullptr, s);
     if (rc) return rc;
-    if (h->cfg.out_mode == SDBE_OUT_FRAMES) {
-        HdrDesc hd;
-        hd.secs = h->st.secs;
-        hd.df = h->st.df;
-        hd.ref_epoch = h->st.ref_epoch;
-        hd.psn = h->st.psn;
-        hd.station = h->cfg.station_id;
-        hd.bdc = h->cfg.bdc_sideband;
-        hd.rx = h->cfg.rx_sideband;
-        hd.n_frames_per_pol = n_bp * OUT_FRAMES_BLOCK;
-        hd.n_pols = 1;
-        LAUNCH(h, KID_HEADERS, n_bp, s, k_headers, dim3((hd.n_frames_per_pol + 255) / 256), dim3(256), od, hd);
-        HIP_TRY(h, hipGetLastError());
-    }
     advance_state(h->st, (int64_t)n_bp * OUT_FRAMES_BLOCK);
     h->n_packets_last = 0;
     h->n_bp_last = n_bp;
@@ -1190,17 +1294,129 @@ int sdbe_resample_spectra_f32(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_
     if (n_bp > h->cfg.max_spectra_blockpols) return SDBE_ERR_CAPACITY;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    SrcDesc sd = {d_beng, 0};
-    OutDesc od;
-    memset(&od, 0, sizeof(od));
-    od.f32 = d_samples;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h->cnt, CNT_N, 0);
-    int rc = run_resampler(h, sd, n_bp, od, d_stats, s);
+    int rc = spectra_call(h, d_beng, n_bp, (uint8_t *)d_samples, (int64_t)P20 * 2 * sizeof(float), true, d_stats,
+                          s);
     if (rc) return rc;
     h->n_packets_last = 0;
     h->n_bp_last = n_bp;
     h->have_group = 0;
     return finish_call(h, s);
+}
+
+namespace {
+// Descriptors of a ragged batch: stream by stream, block by block, pol 0 then 1.
+int fill_batch(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int32_t n_streams, BpDesc *hb) {
+    const OutDesc od = out_desc(h);
+    int n = 0;
+    for (int i = 0; i < n_streams; ++i) {
+        const sdbe_stream_desc_t &d = streams[i];
+        for (int b = 0; b < d.n_blocks; ++b)
+            for (int pol = 0; pol < 2; ++pol) {
+                uint8_t *out = (uint8_t *)d.d_out + (int64_t)pol * d.out_pol_stride_bytes +
+                               (int64_t)b * OUT_FRAMES_BLOCK * od.frame_bytes;
+                set_bp(hb[n++], d.d_beng[pol] + (size_t)b * NBLK * NCH * 2, out,
+                       state_plus(*d.state, (int64_t)b * OUT_FRAMES_BLOCK), pol, d.flip, d.thresh_policy,
+                       d.latch_slot, d.fixed_threshold, d.fixed_offset, d.station_id, d.bdc_sideband,
+                       d.rx_sideband);
+            }
+    }
+    return n;
+}
+
+int check_batch(const sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int32_t n_streams, int &n_bp) {
+    if (!streams || n_streams <= 0) return SDBE_ERR_ARG;
+    const OutDesc od = out_desc(h);
+    int64_t total = 0;
+    for (int i = 0; i < n_streams; ++i) {
+        const sdbe_stream_desc_t &d = streams[i];
+        if (d.n_blocks < 0 || !d.state || (d.n_blocks > 0 && (!d.d_beng[0] || !d.d_beng[1] || !d.d_out)))
+            return SDBE_ERR_ARG;
+        if (d.thresh_policy < SDBE_THRESH_REFERENCE || d.thresh_policy > SDBE_THRESH_FIXED) return SDBE_ERR_ARG;
+        if (d.latch_slot < 0 || d.latch_slot >= SDBE_LATCH_SLOTS) return SDBE_ERR_ARG;
+        if (d.out_pol_stride_bytes < (int64_t)d.n_blocks * OUT_FRAMES_BLOCK * od.frame_bytes) return SDBE_ERR_ARG;
+        total += 2 * (int64_t)d.n_blocks;
+    }
+    if (total > h->cap_bp) return SDBE_ERR_CAPACITY;
+    n_bp = (int)total;
+    return SDBE_OK;
+}
+
+int batch_graph(sdbe_handle_t *h, int n_bp) {
+    if (h->gexec && h->g_nbp == n_bp) return SDBE_OK;
+    if (h->gexec) {
+        HIP_TRY(h, hipEventSynchronize(h->g_done));
+        HIP_TRY(h, hipGraphExecDestroy(h->gexec));
+        h->gexec = nullptr;
+    }
+    if (!h->cap_stream) HIP_TRY(h, hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+    hipStream_t cs = h->cap_stream;
+    HIP_TRY(h, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    h->capturing = 1;
+    hipGraph_t g = nullptr;
+    hipError_t e1 = hipMemcpyAsync(h->g_d, h->g_h, (size_t)n_bp * sizeof(BpDesc), hipMemcpyHostToDevice, cs);
+    if (e1 == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, cs, h->cnt, CNT_N, 0);
+        e1 = hipGetLastError();
+    }
+    int rc = e1 == hipSuccess ? run_resampler(h, h->g_d, n_bp, out_desc(h), false, nullptr, cs) : SDBE_OK;
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    h->capturing = 0;
+    if (e1 != hipSuccess) return hip_fail(h, e1);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e2 != hipSuccess) return hip_fail(h, e2);
+    hipError_t e3 = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) {
+        h->gexec = nullptr;
+        return hip_fail(h, e3);
+    }
+    h->g_nbp = n_bp;
+    return SDBE_OK;
+}
+}  // namespace
+
+int sdbe_process_blocks(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int32_t n_streams, void *stream) {
+    if (!h) return SDBE_ERR_ARG;
+    int n_bp = 0;
+    int rc = check_batch(h, streams, n_streams, n_bp);
+    if (rc) return rc;
+    HIP_TRY(h, hipSetDevice(h->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (n_bp > 0) {
+        if (h->cfg.use_graph) {
+            if ((rc = batch_graph(h, n_bp))) return rc;
+            // the graph copies g_h in: wait until its previous launch has done so
+            if (h->g_used) HIP_TRY(h, hipEventSynchronize(h->g_done));
+            fill_batch(h, streams, n_streams, h->g_h);
+            HIP_TRY(h, hipGraphLaunch(h->gexec, s));
+            HIP_TRY(h, hipEventRecord(h->g_done, s));
+            h->g_used = 1;
+        } else {
+            int slot;
+            if ((rc = ring_acquire(h, slot))) return rc;
+            fill_batch(h, streams, n_streams, h->h_bpd[slot]);
+            if ((rc = ring_upload(h, slot, n_bp, s))) return rc;
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h->cnt, CNT_N, 0);
+            if ((rc = run_resampler(h, h->d_bpd[slot], n_bp, out_desc(h), false, nullptr, s))) return rc;
+            if ((rc = ring_release(h, slot, s))) return rc;
+        }
+    }
+    for (int i = 0; i < n_streams; ++i) advance_state(*streams[i].state, (int64_t)streams[i].n_blocks * OUT_FRAMES_BLOCK);
+    h->n_packets_last = 0;
+    h->n_bp_last = n_bp;
+    h->have_group = 0;
+    return finish_call(h, s);
+}
+
+int sdbe_reset_latch(sdbe_handle_t *h, int32_t slot, void *stream) {
+    if (!h || slot < 0 || slot >= SDBE_LATCH_SLOTS) return SDBE_ERR_ARG;
+    HIP_TRY(h, hipSetDevice(h->device));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, (hipStream_t)stream, (int *)(h->latch + 2 * slot), 2, 0);
+    HIP_TRY(h, hipGetLastError());
+    return SDBE_OK;
 }
 
 int sdbe_get_stats(sdbe_handle_t *h, sdbe_stats_t *o) {

@@ -31,7 +31,8 @@ def station_code(s):
 
 class SdbeStage:
     def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=128, device=0,
-                 station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0)):
+                 station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0), use_graph=False,
+                 c1_tiles_per_wg=0, max_batch=0):
         self.lib = L.load()
         cfg = L.Config()
         L.check(self.lib.sdbe_default_config(C.byref(cfg)), "sdbe_default_config")
@@ -44,6 +45,9 @@ class SdbeStage:
         cfg.rx_sideband = rx
         cfg.fixed_threshold = fixed[0]
         cfg.fixed_offset = fixed[1]
+        cfg.use_graph = 1 if use_graph else 0
+        cfg.c1_tiles_per_wg = c1_tiles_per_wg
+        cfg.max_batch_blockpols = max_batch
         self.cfg = cfg
         self.device = device
         self.out_mode = out_mode
@@ -77,6 +81,9 @@ class SdbeStage:
         s = L.Stats()
         L.check(self.lib.sdbe_get_stats(self.h, C.byref(s)), "sdbe_get_stats", self.h)
         return {k: getattr(s, k) for k, _ in L.Stats._fields_ if k != "reserved"}
+
+    def reset_latch(self, slot, stream=None):
+        L.check(self.lib.sdbe_reset_latch(self.h, slot, _stream(stream)), "sdbe_reset_latch", self.h)
 
     def profile(self, enable=True):
         L.check(self.lib.sdbe_profile_enable(self.h, 1 if enable else 0), "sdbe_profile_enable", self.h)
@@ -146,6 +153,51 @@ class SdbeStage:
                                                 _stream(stream))
         L.check(rc, "sdbe_resample_spectra_f32", self.h)
         return x, st
+
+
+class BatchStream:
+    """One stream of a ragged batch (sdbe_process_blocks): int8 spectra of both
+    polarisations (device tensors [n_blocks * 39][16384][2 = (im, re)]), the
+    lower-sideband flip, threshold policy, REFERENCE latch slot, station
+    fields and VDIF time state (advanced by every call).  `out` receives
+    [2 pols][64 n_blocks][frame bytes]."""
+
+    def __init__(self, beng0, beng1, out, flip=False, policy=L.THRESH_PER_BLOCK, fixed=(1.0, 0.0),
+                 latch_slot=0, station="XX", bdc=0, rx=0, secs=0, df=0, ref_epoch=0, psn=1):
+        assert beng0.shape == beng1.shape and beng0.dtype == torch.int8
+        assert beng0.is_contiguous() and beng1.is_contiguous() and out.is_contiguous()
+        assert beng0.shape[0] % 39 == 0 and tuple(beng0.shape[1:]) == (L.CHANNELS, 2)
+        self.beng = (beng0, beng1)
+        self.out = out
+        self.n_blocks = beng0.shape[0] // 39
+        self.state = L.StreamState(secs=secs, df=df, ref_epoch=ref_epoch, reserved=0, psn=psn)
+        self.desc = L.StreamDesc()
+        d = self.desc
+        d.d_beng[0] = _ptr(beng0)
+        d.d_beng[1] = _ptr(beng1)
+        d.n_blocks = self.n_blocks
+        d.flip = 1 if flip else 0
+        d.d_out = _ptr(out)
+        d.out_pol_stride_bytes = out.numel() // 2
+        d.state = C.pointer(self.state)
+        d.thresh_policy = policy
+        d.fixed_threshold = fixed[0]
+        d.fixed_offset = fixed[1]
+        d.latch_slot = latch_slot
+        d.station_id = station_code(station)
+        d.bdc_sideband = bdc
+        d.rx_sideband = rx
+
+    def get_state(self):
+        st = self.state
+        return dict(secs=st.secs, df=st.df, ref_epoch=st.ref_epoch, psn=st.psn)
+
+
+def process_blocks(stage, streams, stream=None):
+    """Run a ragged batch of BatchStream objects through one sdbe_process_blocks call."""
+    arr = (L.StreamDesc * len(streams))(*[s.desc for s in streams])
+    rc = stage.lib.sdbe_process_blocks(stage.h, arr, len(streams), _stream(stream))
+    L.check(rc, "sdbe_process_blocks", stage.h)   # each desc.state points at its stream's state
 
 
 def state_from_template(template_words, n_words_per_chan=L.OUT_FRAMES_PER_GROUP * 2048):

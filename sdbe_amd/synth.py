@@ -43,14 +43,17 @@ def quantize_components(x, sigma_c):
     return c
 
 
-def gaussian_codes(rng, n_snap, sigma=1.0, tones=((5000, 0.01, 0.0371),)):
+def gaussian_codes(rng, n_snap, sigma=1.0, tones=((5000, 0.01, 0.0371),), quant_sigma=None):
     """Codes [n_snap][16384][2 = (re, im)] for CN(0, sigma^2) noise plus tones.
 
     ``tones`` holds (channel, power relative to sigma^2, phase step per
-    snapshot in cycles).  Noise-only channels are drawn by inverse CDF from
-    uniforms (same distribution as quantising Gaussian samples); tone channels
-    are drawn as Gaussians plus the coherent tone and then quantised."""
-    p0 = _phi(-THRESH_SIGMA)
+    snapshot in cycles).  The quantiser thresholds are +-0.9816 quant_sigma_c
+    (quant_sigma defaults to sigma: a quantiser matched to the signal).
+    Noise-only channels are drawn by inverse CDF from uniforms (same
+    distribution as quantising Gaussian samples); tone channels are drawn as
+    Gaussians plus the coherent tone and then quantised."""
+    qs = sigma if quant_sigma is None else quant_sigma
+    p0 = _phi(-THRESH_SIGMA * qs / sigma)
     u = rng.random((n_snap, N_CHAN, 2), dtype=np.float32)
     codes = np.full(u.shape, 3, dtype=np.uint8)
     codes[u < 1.0 - p0] = 2
@@ -63,8 +66,26 @@ def gaussian_codes(rng, n_snap, sigma=1.0, tones=((5000, 0.01, 0.0371),)):
         z = rng.normal(0.0, sigma_c, (n_snap, 2))
         z[:, 0] += amp * np.cos(phase)
         z[:, 1] += amp * np.sin(phase)
-        codes[:, ch, :] = quantize_components(z, sigma_c)
+        codes[:, ch, :] = quantize_components(z, qs * math.sqrt(0.5))
     return codes
+
+
+def batch_streams(seed, n_blocks=(17, 64, 128, 5, 200, 33), lsb=(False, True, False, True, False, True)):
+    """Ragged multi-stream input (3 beams x 2 sidebands): per stream int8
+    spectra [2 pols][39 n][16384][2 = (im, re)], sigma ~ U[0.5, 2] quantised
+    by a unit-sigma 2-bit quantiser, a random tone per stream.  Returns a list
+    of dicts (beng, n_blocks, lsb, sigma, tone_channel)."""
+    rng = rng_for(seed)
+    out = []
+    for nb, lo in zip(n_blocks, lsb):
+        sigma = float(rng.uniform(0.5, 2.0))
+        ch = int(rng.integers(100, N_CHAN - 100))
+        step = float(rng.uniform(-0.5, 0.5))
+        tones = ((ch, float(rng.uniform(0.005, 0.05)), step),)
+        pols = [codes_to_beng_int8(gaussian_codes(rng, 39 * nb, sigma, tones, quant_sigma=1.0))
+                for _ in range(2)]
+        out.append(dict(beng=np.stack(pols), n_blocks=nb, lsb=lo, sigma=sigma, tone_channel=ch))
+    return out
 
 
 def codes_to_beng_int8(codes):

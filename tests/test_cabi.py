@@ -44,6 +44,11 @@ int main(void) {
   printf("sdbe_stream_state_t %zu\\n", sizeof(sdbe_stream_state_t));
   printf("sdbe_stats_t %zu\\n", sizeof(sdbe_stats_t));
   printf("sdbe_kernel_time_t %zu\\n", sizeof(sdbe_kernel_time_t));
+  printf("sdbe_stream_desc_t %zu\\n", sizeof(sdbe_stream_desc_t));
+  P(sdbe_config_t, use_graph) P(sdbe_config_t, max_batch_blockpols)
+  P(sdbe_stream_desc_t, n_blocks) P(sdbe_stream_desc_t, d_out) P(sdbe_stream_desc_t, state)
+  P(sdbe_stream_desc_t, thresh_policy) P(sdbe_stream_desc_t, latch_slot) P(sdbe_stream_desc_t, station_id)
+  P(sdbe_stream_desc_t, rx_sideband) P(sdbe_stream_desc_t, reserved)
   P(sdbe_config_t, station_id) P(sdbe_config_t, rx_sideband) P(sdbe_config_t, reserved)
   P(sdbe_stream_state_t, psn) P(sdbe_stats_t, latched_threshold) P(sdbe_kernel_time_t, total_ms)
   return 0;
@@ -63,6 +68,12 @@ int main(void) {
     assert int(out["sdbe_stream_state_t.psn"]) == L.StreamState.psn.offset
     assert int(out["sdbe_stats_t.latched_threshold"]) == L.Stats.latched_threshold.offset
     assert int(out["sdbe_kernel_time_t.total_ms"]) == L.KernelTime.total_ms.offset
+    assert int(out["sdbe_stream_desc_t"]) == C.sizeof(L.StreamDesc)
+    assert int(out["sdbe_config_t.use_graph"]) == L.Config.use_graph.offset
+    assert int(out["sdbe_config_t.max_batch_blockpols"]) == L.Config.max_batch_blockpols.offset
+    for f in ("n_blocks", "d_out", "state", "thresh_policy", "latch_slot", "station_id", "rx_sideband",
+              "reserved"):
+        assert int(out["sdbe_stream_desc_t." + f]) == getattr(L.StreamDesc, f).offset, f
 
 
 def test_host_only_entry_points_without_gpu():

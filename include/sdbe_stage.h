@@ -129,7 +129,7 @@ typedef struct sdbe_stats {
 typedef struct sdbe_handle sdbe_handle_t;
 
 /* Fill `cfg` with the reference defaults (REFERENCE policy, payload output,
- * station "XX", chunk of 128 block-pols).                                  */
+ * station "XX", chunk of 256 block-pols).                                  */
 int sdbe_default_config(sdbe_config_t *cfg);
 
 /* Allocate plans, twiddle tables and scratch on HIP device `device`.        */

@@ -736,3 +736,96 @@ def cpu_chain(pkts, b_zero, blocks, workers=1, state=None):
         words.append(pack_codes(quantize2bit(x[i], sig, mean)))
     hdr, _ = stamp_headers(len(blocks) * OUT_FRAMES_PER_BLOCK, state or StreamState())
     return np.stack(words), hdr
+
+
+# ---------------------------------------------------------------------------
+# A17: polyphase filterbank and its least-squares inverse
+# (software/prototyping/cuda/pfb_inverse.cu; the inverse follows the published
+# method the file names -- Richard Shaw's pfb-inverse -- whose solve the CUDA
+# file leaves compiled out, :392-455).
+# ---------------------------------------------------------------------------
+PFB_LBLOCK = 2 * BENG_CHANNELS      # 32768 samples per block (num_freq 16385)
+PFB_NTAP = 4
+
+
+def pfb_window_hamming(ntap=PFB_NTAP, lblock=PFB_LBLOCK):
+    """The reference window over ntap * lblock samples
+    (pfb_inverse.cu:51-53, used at :193 and :271)."""
+    m = ntap * lblock
+    n = np.arange(m, dtype=np.float64)
+    return 0.54 - 0.46 * np.cos(2.0 * np.pi * n / (m - 1.0))
+
+
+def pfb_window_hann_sinc(ntap=PFB_NTAP, lblock=PFB_LBLOCK):
+    """Hann-windowed sinc prototype: h[n] = w_Hann[n] sinc((n - (L-1)/2) / lblock)."""
+    m = ntap * lblock
+    n = np.arange(m, dtype=np.float64)
+    return (0.5 - 0.5 * np.cos(2.0 * np.pi * n / (m - 1.0))) * np.sinc((n - (m - 1) / 2.0) / lblock)
+
+
+def pfb_forward(x, window, lblock=PFB_LBLOCK):
+    """Spectra of block i: rfft of the window times samples
+    [i lblock, i lblock + ntap lblock), keeping every ntap-th bin
+    (pfb_inverse.cu:172-208).  Computed here as the rfft of the ntap-fold
+    sum, which equals those bins.  Returns (nblock, lblock/2 + 1) complex."""
+    x = np.asarray(x, dtype=np.float64)
+    ntap = window.size // lblock
+    nblock = x.size // lblock - (ntap - 1)
+    w = window.reshape(ntap, lblock)
+    seg = np.stack([x[p * lblock:(p + nblock) * lblock].reshape(nblock, lblock) for p in range(ntap)])
+    folded = np.einsum("pil,pl->il", seg, w)
+    return np.fft.rfft(folded, axis=-1)
+
+
+def pfb_toeplitz(window, lblock=PFB_LBLOCK):
+    """a[d][n] = sum_k w[n + lblock (k + d)] w[n + lblock k]: the diagonals of
+    P P^T for branch n, P[i][j] = w[n + lblock (j - i)] for 0 <= j - i < ntap
+    (pfb_inverse.cu:286-309)."""
+    ntap = window.size // lblock
+    w = window.reshape(ntap, lblock).astype(np.float64)
+    return np.stack([(w[d:] * w[:ntap - d]).sum(axis=0) for d in range(ntap)])
+
+
+def pfb_inverse(spec, window, lblock=PFB_LBLOCK):
+    """Least-squares (minimum-norm) inverse: yh_i = irfft(spec_i) (the
+    pseudo-timestream, pfb_inverse.cu:355-358 with the 1/lblock scaling of
+    :377), then per branch n solve (P P^T) z = yh[:, n] and project
+    x[j lblock + n] = (P^T z)[j] = sum_p w[n + lblock p] z[j - p]
+    (pfb_inverse.cu:380-400).  The banded SPD system is solved by Cholesky,
+    vectorised over the branches.  Returns ((nblock + ntap - 1) * lblock,)."""
+    yh = np.fft.irfft(np.asarray(spec), n=lblock, axis=-1)
+    nblock = yh.shape[0]
+    ntap = window.size // lblock
+    a = pfb_toeplitz(window, lblock)
+    w = window.reshape(ntap, lblock).astype(np.float64)
+    b = ntap - 1
+    L = np.zeros((nblock, ntap, lblock))         # L[i][d] = L(i, i - d)
+    u = np.zeros((nblock, lblock))
+    for i in range(nblock):
+        for d in range(b, 0, -1):
+            if i - d < 0:
+                continue
+            acc = a[d].copy()
+            for k in range(d + 1, b + 1):
+                if i - k >= 0:
+                    acc -= L[i][k] * L[i - d][k - d]
+            L[i][d] = acc / L[i - d][0]
+        diag = a[0] - sum(L[i][d] ** 2 for d in range(1, b + 1))
+        L[i][0] = np.sqrt(diag)
+        r = yh[i].copy()
+        for d in range(1, b + 1):
+            if i - d >= 0:
+                r -= L[i][d] * u[i - d]
+        u[i] = r / L[i][0]
+    z = np.zeros((nblock, lblock))
+    for i in range(nblock - 1, -1, -1):
+        r = u[i].copy()
+        for d in range(1, b + 1):
+            if i + d < nblock:
+                r -= L[i + d][d] * z[i + d]
+        z[i] = r / L[i][0]
+    nts = nblock + ntap - 1
+    x = np.zeros((nts, lblock))
+    for p in range(ntap):
+        x[p:p + nblock] += w[p] * z
+    return x.reshape(-1)

@@ -110,7 +110,12 @@ class KernelTime(C.Structure):
     ]
 
 
+PFB_LBLOCK = 32768
+PFB_NBINS = 16385
+PFB_NTAP = 4
+
 # (name, restype, argtypes) of every symbol declared in include/sdbe_stage.h
+# and include/sdbe_pfb.h
 PROTOTYPES = [
     ("sdbe_default_config", C.c_int, [C.POINTER(Config)]),
     ("sdbe_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
@@ -131,6 +136,12 @@ PROTOTYPES = [
     ("sdbe_process_blocks", C.c_int, [C.c_void_p, C.POINTER(StreamDesc), C.c_int32, C.c_void_p]),
     ("sdbe_reset_latch", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("sdbe_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    # include/sdbe_pfb.h
+    ("sdbe_pfb_create", C.c_int, [C.c_int, C.c_int32, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("sdbe_pfb_destroy", None, [C.c_void_p]),
+    ("sdbe_pfb_last_hip_error", C.c_int, [C.c_void_p]),
+    ("sdbe_pfb_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("sdbe_pfb_inverse", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("sdbe_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("sdbe_profile_read", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int32, C.POINTER(C.c_int32)]),
 ]

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 namespace sdbe {
@@ -229,6 +230,20 @@ constexpr int FFT_TW_N = 640;
 __device__ __forceinline__ void fft_tw_load(FftTw *L, const float2 *g) {
     float2 *d = reinterpret_cast<float2 *>(L);
     for (int i = threadIdx.x; i < FFT_TW_N; i += blockDim.x) d[i] = g[i];
+}
+
+// Host: the FftTw image (FFT_TW_N entries, layout of FftTw), computed in FP64.
+inline void fft_tw_fill(float2 *host) {
+    const double two_pi = 6.283185307179586476925286766559;
+    auto e = [&](double num, double den) {
+        return make_float2((float)cos(two_pi * num / den), (float)sin(two_pi * num / den));
+    };
+    float2 *p = host;
+    for (int m = 0; m < 256; ++m) *p++ = e(m, 256);
+    for (int m = 0; m < 64; ++m) *p++ = e(64.0 * m, 4096);
+    for (int m = 0; m < 64; ++m) *p++ = e(m, 4096);
+    for (int m = 0; m < 128; ++m) *p++ = e(128.0 * m, 16384);
+    for (int m = 0; m < 128; ++m) *p++ = e(m, 16384);
 }
 template <int S> __device__ __forceinline__ float2 ctw(float2 w) { return S > 0 ? w : cconj(w); }
 

@@ -799,17 +799,6 @@ void tw_fill(float2 *host, const TwSpec &s) {
     }
 }
 
-void fft_tw_fill(float2 *host) {
-    const double two_pi = 6.283185307179586476925286766559;
-    auto e = [&](double num, double den) { return make_float2((float)cos(two_pi * num / den), (float)sin(two_pi * num / den)); };
-    float2 *p = host;
-    for (int m = 0; m < 256; ++m) *p++ = e(m, 256);
-    for (int m = 0; m < 64; ++m) *p++ = e(64.0 * m, 4096);
-    for (int m = 0; m < 64; ++m) *p++ = e(m, 4096);
-    for (int m = 0; m < 128; ++m) *p++ = e(128.0 * m, 16384);
-    for (int m = 0; m < 128; ++m) *p++ = e(m, 16384);
-}
-
 int build_tables(sdbe_handle_t *h) {
     const TwSpec specs[6] = {
         {16384, 14, 0},                 // unused slot (the FFT image is stored separately)
@@ -1006,7 +995,7 @@ int sdbe_default_config(sdbe_config_t *cfg) {
     cfg->fixed_threshold = 1.0f;
     cfg->fixed_offset = 0.0f;
     cfg->out_mode = SDBE_OUT_PAYLOAD;
-    cfg->chunk_blockpols = 128;
+    cfg->chunk_blockpols = 256;
     cfg->max_spectra_blockpols = 256;
     cfg->station_id = (uint16_t)((('X' & 0xFF) << 8) | ('X' & 0xFF));
     return SDBE_OK;

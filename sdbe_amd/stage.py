@@ -30,7 +30,7 @@ def station_code(s):
 
 
 class SdbeStage:
-    def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=128, device=0,
+    def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=256, device=0,
                  station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0), use_graph=False,
                  c1_tiles_per_wg=0, max_batch=0):
         self.lib = L.load()

@@ -9,13 +9,15 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HDR = os.path.join(ROOT, "include", "sdbe_stage.h")
+HDRS = [os.path.join(ROOT, "include", h) for h in ("sdbe_stage.h", "sdbe_pfb.h")]
 
 
 def _declared_functions():
-    src = open(HDR).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(sdbe_[a-z0-9_]+)\s*\(", src)))
+    names = set()
+    for h in HDRS:
+        src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(sdbe_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():

@@ -71,7 +71,8 @@ def test_unpack_8pac_matches_oracle(hp):
     raw = rng.integers(0, 256, (3, 8504), dtype=np.uint64).astype(np.uint8)
     want = O.unpack_8pac(raw)
     got = np.zeros((24, O.PKT_WORDS), dtype=np.uint32)
-    hp.unpack_8pac(got.ctypes.data, raw.copy().ctypes.data, 3)
+    src = raw.copy()              # unpack_8pac rewrites the source headers in place: keep it alive
+    hp.unpack_8pac(got.ctypes.data, src.ctypes.data, 3)
     assert np.array_equal(got, want)
 
 

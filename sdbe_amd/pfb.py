@@ -69,3 +69,49 @@ class Pfb:
         x = torch.empty(((n + L.PFB_NTAP - 1) * L.PFB_LBLOCK,), dtype=torch.float32, device=spec.device)
         self._check(self.lib.sdbe_pfb_inverse(self.h, _ptr(spec), n, _ptr(x), _stream(stream)), "sdbe_pfb_inverse")
         return x
+
+
+def main():
+    """Time the forward PFB and the inverse on n spectra (default 4096, the
+    16384-channel config of 4096 spectra) of white noise with the Hann-windowed
+    sinc prototype; prints one JSON line."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--spectra", type=int, default=4096)
+    ap.add_argument("--reps", type=int, default=10)
+    args = ap.parse_args()
+    n, lb = args.spectra, L.PFB_LBLOCK
+    m = L.PFB_NTAP * lb
+    k = np.arange(m, dtype=np.float64)
+    window = (0.5 - 0.5 * np.cos(2 * np.pi * k / (m - 1))) * np.sinc((k - (m - 1) / 2) / lb)
+    p = Pfb(window, max_spectra=n)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(((n + 3) * lb,), generator=g, device="cuda", dtype=torch.float32)
+    spec = p.forward(x)
+    xr = p.inverse(spec)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        out = []
+        for _ in range(args.reps):
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            out.append(e0.elapsed_time(e1))
+        return float(np.median(out))
+
+    t_fwd = timed(lambda: p.forward(x))
+    t_inv = timed(lambda: p.inverse(spec))
+    samples = (n + 3) * lb
+    print(json.dumps(dict(kind="pfb", spectra=n, samples=samples, forward_ms=round(t_fwd, 3),
+                          inverse_ms=round(t_inv, 3), forward_gsample_s=round(samples / t_fwd * 1e-6, 2),
+                          inverse_gsample_s=round(samples / t_inv * 1e-6, 2))))
+    del xr
+    p.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -330,7 +330,7 @@ __device__ __forceinline__ void c1_fetch(const float2 *__restrict__ Zf, const Ta
 }
 
 __global__ void __launch_bounds__(256)
-k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__restrict__ part, Tables tb, int nb,
+k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__restrict__ part, Tables tb, int nb,
            int tpw) {
     __shared__ float2 sz[32][41];
     __shared__ float2 sy[32][41];
@@ -421,12 +421,17 @@ k_stage_c1(const float2 *__restrict__ Zf, float2 *__restrict__ Bc, double *__res
             for (int r = 0; r < 8; ++r) sc[col][j + 8 * r] = v[r];
         }
         __syncthreads();
-        // write the tile's 32 slots of every row tx (256-byte runs)
-        float2 *bc = Bc + (size_t)x.bpl * KHI * BC_ROW + (size_t)x.tile * 32;
+        // write the tile's 32 slots of every row tx: Bc rows are planar (real
+        // plane then imaginary plane, BC_ROW floats each): two 128-byte runs
+        float *bc = Bc + (size_t)x.bpl * KHI * 2 * BC_ROW + (size_t)x.tile * 32;
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             const int tx = it * 8 + rw;
-            if (x.valid && !x.dup) bc[(size_t)tx * BC_ROW + col] = sc[col][tx];
+            if (x.valid && !x.dup) {
+                const float2 y = sc[col][tx];
+                bc[(size_t)tx * 2 * BC_ROW + col] = y.x;
+                bc[(size_t)tx * 2 * BC_ROW + BC_ROW + col] = y.y;
+            }
         }
         // Parseval partial: wave reduce then cross-wave
 #pragma unroll
@@ -537,7 +542,7 @@ struct OutDesc {
 // the 4-bit pair code (re | im << 2) of t goes to stage[bp][tx][ty].
 template <bool F32>
 __global__ void __launch_bounds__(1024)
-k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, const BpDesc *__restrict__ bpd,
+k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const BpDesc *__restrict__ bpd,
            uint8_t *__restrict__ stage, const int *__restrict__ slot_col, Tables tb) {
     __shared__ float lre[LDS_PAD_N];
     __shared__ float lim[LDS_PAD_N];
@@ -545,46 +550,41 @@ k_stage_c2(const float2 *__restrict__ Bc, const float2 *__restrict__ thr, const 
     fft_tw_load(&ftw, tb.fft);
     const int tx = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
     const float scale = 1.4551915228366852e-11f;   // 2^-36, exact
-    // the row arrives in stage C1's tile-major slot order: scatter it into LDS in
-    // column order (slot -> column table, -1 for slots without a column)
-    // (all loads are issued before any LDS store; slots without a column are
-    // read too -- the row is allocated -- and dropped)
-    const float2 *src = Bc + ((size_t)bpl * KHI + tx) * BC_ROW;
+    // The row arrives in stage C1's tile-major slot order (planar: real plane
+    // then imaginary plane) and is scattered into LDS in column order: slot
+    // 32 tile + col holds column c1_column(tile, col) when that is valid and not
+    // a duplicate.  All loads are issued before any LDS store; slots without a
+    // column are read too (the row is allocated) and dropped.
+    (void)slot_col;
+    const float *src = Bc + ((size_t)bpl * KHI + tx) * 2 * BC_ROW;
     constexpr int NSLOT = BC_ROW / 1024;          // 17 full passes ...
     constexpr int REM = BC_ROW - NSLOT * 1024;    // ... and 64 slots left
+    const int rd = t + (t >> 4);
     {
-        float2 val[NSLOT];
-        int cc[NSLOT];
+        float pr[NSLOT + 1], pi[NSLOT + 1];
 #pragma unroll
         for (int i = 0; i < NSLOT; ++i) {
-            val[i] = src[t + 1024 * i];
-            cc[i] = slot_col[t + 1024 * i];
+            pr[i] = src[t + 1024 * i];
+            pi[i] = src[BC_ROW + t + 1024 * i];
         }
-        float2 vr = make_float2(0.f, 0.f);
-        int cr = -1;
-        if (t < REM) {
-            vr = src[t + 1024 * NSLOT];
-            cr = slot_col[t + 1024 * NSLOT];
-        }
+        pr[NSLOT] = t < REM ? src[t + 1024 * NSLOT] : 0.f;
+        pi[NSLOT] = t < REM ? src[BC_ROW + t + 1024 * NSLOT] : 0.f;
 #pragma unroll
-        for (int i = 0; i < NSLOT; ++i) {
-            if (cc[i] >= 0) {
-                lre[cc[i] + (cc[i] >> 4)] = val[i].x;
-                lim[cc[i] + (cc[i] >> 4)] = val[i].y;
+        for (int i = 0; i <= NSLOT; ++i) {
+            const int slot = t + 1024 * i;
+            int c;
+            bool valid, dup;
+            c1_column(slot >> 5, slot & 31, c, valid, dup);
+            if ((i < NSLOT || t < REM) && valid && !dup) {
+                lre[c + (c >> 4)] = pr[i];
+                lim[c + (c >> 4)] = pi[i];
             }
-        }
-        if (cr >= 0) {
-            lre[cr + (cr >> 4)] = vr.x;
-            lim[cr + (cr >> 4)] = vr.y;
         }
     }
     __syncthreads();
     float2 v[16];
-    {
-        const int rd = t + (t >> 4);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
-    }
+    for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
     // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
     // times a workgroup-uniform one
     const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
@@ -877,15 +877,15 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
         LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
                (nb % 8) == 0 ? 1 : 0);
         const int c1_wgs = (C1_TILES * nb + h->c1_tpw - 1) / h->c1_tpw;
-        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(c1_wgs), dim3(256), (const float2 *)h->Zf, h->Bc, h->part, h->tb,
+        LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(c1_wgs), dim3(256), (const float2 *)h->Zf, (float *)h->Bc, h->part, h->tb,
                nb, h->c1_tpw);
         LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
         LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(256), (const double2 *)h->ms, nb, bpd, h->latch, h->thr);
         if (f32) {
-            LAUNCH(h, KID_C2, nb, s, k_stage_c2<true>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
+            LAUNCH(h, KID_C2, nb, s, k_stage_c2<true>, dim3(KHI, nb), dim3(1024), (const float *)h->Bc,
                    (const float2 *)h->thr, bpd, h->stage, (const int *)h->slot_col, h->tb);
         } else {
-            LAUNCH(h, KID_C2, nb, s, k_stage_c2<false>, dim3(KHI, nb), dim3(1024), (const float2 *)h->Bc,
+            LAUNCH(h, KID_C2, nb, s, k_stage_c2<false>, dim3(KHI, nb), dim3(1024), (const float *)h->Bc,
                    (const float2 *)h->thr, bpd, h->stage, (const int *)h->slot_col, h->tb);
             LAUNCH(h, KID_PACK, nb, s, k_pack, dim3((NCH / 64) * nb), dim3(256), (const uint8_t *)h->stage, bpd, nb,
                    od);

@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--policy", default="per_block", choices=["per_block", "reference"])
+    ap.add_argument("--precision", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="FFT arithmetic: 0 FP32 (default), 1 BF16X3, 2 BF16, 3 FP16 matrix-core passes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
     args = ap.parse_args()
@@ -134,7 +136,8 @@ def main():
     groups, _ = synth.make_group_packets(seed=args.seed, n_groups=2, drop_frac=0.001)
     n_pkts = sum(pk.shape[0] for pk, _ in groups)
     policy = L.THRESH_PER_BLOCK if args.policy == "per_block" else L.THRESH_REFERENCE
-    st = SdbeStage(policy=policy, out_mode=L.OUT_FRAMES, chunk=args.chunk, station="Sm")
+    st = SdbeStage(policy=policy, out_mode=L.OUT_FRAMES, chunk=args.chunk, station="Sm",
+                   fft_precision=args.precision)
     st.set_state(secs=1000, df=0, ref_epoch=30, psn=1)
     dpk = [st.to_device_packets(pk) for pk, _ in groups]
     outs = [st.group_out() for _ in groups]
@@ -192,7 +195,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": ["f32", "f32+bf16x3-mfma", "f32+bf16-mfma", "f32+f16-mfma"][args.precision],
         "data": "synthetic",
         "config": {
             "workload": "plain FP32 pipeline at full rate: 256 blocks x 2 pols x 16384 ch "
@@ -201,6 +204,7 @@ def main():
             "packets_per_step": int(n_pkts),
             "threshold_policy": args.policy,
             "chunk_blockpols": args.chunk,
+            "fft_precision": ["fp32", "bf16x3", "bf16", "fp16"][args.precision],
             "parallelism": "1 GPU",
         },
         "roofline": {

@@ -76,6 +76,18 @@ extern "C" {
 #define SDBE_THRESH_PER_BLOCK 1
 #define SDBE_THRESH_FIXED     2
 
+/* ---- FFT arithmetic (precision variants) -------------------------------- */
+/* FP32: radix-16 butterflies on the vector ALUs in FP32 (the default).
+ * BF16X3: radix-16 passes as DFT-as-GEMM on the matrix cores, each operand
+ *   split into bf16 high + low parts, three products, FP32 accumulation.
+ * BF16 / FP16: one bf16 / f16 product per pass (FP16 with a power-of-two
+ *   scale per DFT column), FP32 accumulation.
+ * Everything else (twiddles, the other stages, statistics) is unchanged.   */
+#define SDBE_FFT_FP32   0
+#define SDBE_FFT_BF16X3 1
+#define SDBE_FFT_BF16   2
+#define SDBE_FFT_FP16   3
+
 /* ---- output layout ------------------------------------------------------ */
 /* PAYLOAD: 8192-byte payloads back to back per polarisation, the layout of
  *   vdif_out_data_group_t (software/aphids/vdif_out_databuf.h:110-116).
@@ -99,7 +111,9 @@ typedef struct sdbe_config {
                                    and replay it                               */
     int32_t c1_tiles_per_wg;    /* tuning: stage-C1 tiles per workgroup (1..64, 0 = 8) */
     int32_t max_batch_blockpols;/* capacity of sdbe_process_blocks (0 = 1024)   */
-    int32_t reserved[5];
+    int32_t fft_precision;      /* SDBE_FFT_*: arithmetic of the 16384-point FFT
+                                   passes of stages B and C2                   */
+    int32_t reserved[4];
 } sdbe_config_t;
 
 /* VDIF output time state for the next frame to be produced; both

@@ -21,6 +21,10 @@ SDBE_ERR_STATE = -6
 THRESH_REFERENCE = 0
 THRESH_PER_BLOCK = 1
 THRESH_FIXED = 2
+FFT_FP32 = 0
+FFT_BF16X3 = 1
+FFT_BF16 = 2
+FFT_FP16 = 3
 OUT_PAYLOAD = 0
 OUT_FRAMES = 1
 
@@ -49,7 +53,8 @@ class Config(C.Structure):
         ("use_graph", C.c_int32),
         ("c1_tiles_per_wg", C.c_int32),
         ("max_batch_blockpols", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("fft_precision", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

@@ -1,0 +1,213 @@
+// 16384-point FFT whose radix-16 passes run as DFT-as-GEMM on the gfx950
+// matrix cores (v_mfma_f32_16x16x32_{bf16,f16}, FP32 accumulation).
+//
+// A complex DFT-16 in real form is a 32 x 32 matrix (rows: 16 real then 16
+// imaginary outputs; K = 2 r + c over the 16 inputs r and their real / imag
+// parts c) times a 32-row column per DFT.  One 16x16x32 MFMA produces 16 output
+// rows for 16 DFT columns, so a wave transforms 16 DFTs with one MFMA for the
+// real and one for the imaginary outputs (x3 for the split-precision mode).
+//
+// Operand layout of v_mfma_f32_16x16x32 on gfx950 (checked on the device):
+//   A: lane l holds A[l % 16][8 (l / 16) + e],  e < 8
+//   B: lane l holds B[8 (l / 16) + e][l % 16]
+//   C: lane l holds C[4 (l / 16) + i][l % 16],  i < 4
+// so lane (n = l % 16, kb = l / 16) supplies inputs r = 4 kb .. 4 kb + 3 of DFT
+// column n and receives outputs k = 4 kb .. 4 kb + 3 of the same column.
+//
+// Work distribution: wave w (of 16), group g (of 4): DFT d = 64 w + 16 g + n.
+// In every radix-16 pass DFT d reads elements d + 1024 r and writes its
+// output k to (d / Ns) 16 Ns + (d % Ns) + Ns k (Stockham, Ns = 1, 16, 256).
+// The last pass (radix 4, Ns = 4096) is the VALU pass of fft16k, so the
+// output is in natural order: register q of thread t holds element t + 1024 q.
+//
+// Precision modes:
+//   FFT_BF16X3  x = hi + lo, W = Whi + Wlo (bf16 each), Y = Whi hi + Whi lo + Wlo hi:
+//               about 2^-16 relative per pass, bf16 keeps the FP32 exponent range
+//   FFT_BF16    one bf16 product (2^-9 relative per pass)
+//   FFT_FP16    one f16 product (2^-12 relative per pass); every DFT column is
+//               scaled by a power of two so that its largest value is below 2^15
+#pragma once
+
+#include "sdbe_device.hpp"
+
+namespace sdbe {
+
+enum FftMode { FFT_BF16X3 = 1, FFT_BF16 = 2, FFT_FP16 = 3 };
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// cos(2 pi m / 16), m = 0..15
+__device__ __forceinline__ float cos16(int m) {
+    const float c[16] = {1.0f, 0.92387953251128674f, 0.70710678118654757f, 0.38268343236508978f,
+                         0.0f, -0.38268343236508978f, -0.70710678118654757f, -0.92387953251128674f,
+                         -1.0f, -0.92387953251128674f, -0.70710678118654757f, -0.38268343236508978f,
+                         0.0f, 0.38268343236508978f, 0.70710678118654757f, 0.92387953251128674f};
+    return c[m & 15];
+}
+
+// Constant DFT-16 operands of one lane for sign S (row k = l % 16).
+template <int S, int P>
+struct MfmaW {
+    bf16x8 re_hi, re_lo, im_hi, im_lo;
+    f16x8 re_h, im_h;
+    __device__ __forceinline__ void init() {
+        const int l = threadIdx.x & 63, k = l & 15, kb = l >> 4;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int r = 4 * kb + (e >> 1);
+            const int m = (r * k) & 15;
+            const float c = cos16(m), s = (S > 0 ? 1.0f : -1.0f) * cos16(m - 4);   // sin(2 pi m/16) = cos(2 pi (m - 4)/16)
+            const float are = (e & 1) ? -s : c;
+            const float aim = (e & 1) ? c : s;
+            if (P == FFT_FP16) {
+                re_h[e] = (_Float16)are;
+                im_h[e] = (_Float16)aim;
+            } else {
+                re_hi[e] = (__bf16)are;
+                im_hi[e] = (__bf16)aim;
+                re_lo[e] = (__bf16)(are - (float)re_hi[e]);
+                im_lo[e] = (__bf16)(aim - (float)im_hi[e]);
+            }
+        }
+    }
+};
+
+// 16 DFT-16s of one wave: x[j] = input 4 kb + j of column n -> output 4 kb + j.
+template <int S, int P>
+__device__ __forceinline__ void mfma_dft16(float2 *x, const MfmaW<S, P> &W) {
+    f32x4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    if (P == FFT_FP16) {
+        float m = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m = fmaxf(m, fmaxf(fabsf(x[j].x), fabsf(x[j].y)));
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        // power-of-two scale putting the column's largest value in [2^13, 2^14)
+        int ex;
+        (void)frexpf(m, &ex);
+        const float sc = m > 0.f ? ldexpf(1.0f, 14 - ex) : 1.0f;
+        f16x8 b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            b[2 * j] = (_Float16)(x[j].x * sc);
+            b[2 * j + 1] = (_Float16)(x[j].y * sc);
+        }
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(W.re_h, b, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(W.im_h, b, ci, 0, 0, 0);
+        const float inv = 1.0f / sc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = make_float2(cr[i] * inv, ci[i] * inv);
+    } else {
+        bf16x8 bh, bl;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            bh[2 * j] = (__bf16)x[j].x;
+            bh[2 * j + 1] = (__bf16)x[j].y;
+            if (P == FFT_BF16X3) {
+                bl[2 * j] = (__bf16)(x[j].x - (float)bh[2 * j]);
+                bl[2 * j + 1] = (__bf16)(x[j].y - (float)bh[2 * j + 1]);
+            }
+        }
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.re_hi, bh, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.im_hi, bh, ci, 0, 0, 0);
+        if (P == FFT_BF16X3) {
+            cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.re_hi, bl, cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.im_hi, bl, ci, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.re_lo, bh, cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.im_lo, bh, ci, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = make_float2(cr[i], ci[i]);
+    }
+}
+
+// Element index of register 4 g + j of this lane in the MFMA layout of a pass
+// input: DFT d = 64 w + 16 g + n, point r = 4 kb + j, element d + 1024 r.
+__device__ __forceinline__ int mfma_elem(int g, int j) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    return 64 * w + 16 * g + (l & 15) + 1024 * (4 * (l >> 4) + j);
+}
+
+__device__ __forceinline__ int lds_pos(int e) { return e + (e >> 4); }
+
+// Radix-16 passes on the matrix cores, then the radix-4 VALU pass.
+// v: the input in MFMA layout (v[4 g + j] = element mfma_elem(g, j));
+// on return v[q] = element threadIdx.x + 1024 q of the transform.
+template <int S, int P>
+__device__ __forceinline__ void fft16k_mfma(float2 *v, float *lre, float *lim, const FftTw *tw,
+                                            const MfmaW<S, P> &W) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, n = l & 15, kb = l >> 4;
+#pragma unroll
+    for (int pass = 0; pass < 3; ++pass) {
+        const int ns = pass == 0 ? 1 : (pass == 1 ? 16 : 256);
+        if (pass > 0) {
+            // twiddle w_d^r, w_d = e^{S 2 pi i (d % Ns) / (16 Ns)}, r = 4 kb + j
+            if (pass == 1) {
+                // d % 16 = n for every group: four twiddles per lane
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float2 tv = ctw<S>(tw->t256[n * (4 * kb + j)]);
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) v[4 * g + j] = cmul(v[4 * g + j], tv);
+                }
+            } else {
+                // w^(4 kb + j) = w^(4 kb) w^j with w = e^{S 2 pi i (d % 256)/4096}
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int m = (64 * w + 16 * g + n) & 255;
+                    const int mb = m * 4 * kb;                   // < 4096
+                    const float2 w1 = ctw<S>(cmul(tw->h4k[m >> 6], tw->l4k[m & 63]));
+                    float2 tj = ctw<S>(cmul(tw->h4k[mb >> 6], tw->l4k[mb & 63]));
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        v[4 * g + j] = cmul(v[4 * g + j], tj);
+                        tj = cmul(tj, w1);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) mfma_dft16<S, P>(v + 4 * g, W);
+        __syncthreads();   // previous readers of the LDS arrays are done
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int d = 64 * w + 16 * g + n;
+            const int base = (d / ns) * 16 * ns + (d % ns);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p = lds_pos(base + ns * (4 * kb + i));
+                lre[p] = v[4 * g + i].x;
+                lim[p] = v[4 * g + i].y;
+            }
+        }
+        __syncthreads();
+        if (pass < 2) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int p = lds_pos(mfma_elem(g, j));
+                    v[4 * g + j] = make_float2(lre[p], lim[p]);
+                }
+        }
+    }
+    // radix 4, Ns = 4096 (as in fft16k): element t + 1024 (b + 4 r) -> v[b + 4 r]
+    const int rd = t + (t >> 4);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t j = (uint32_t)(t + 1024 * b);
+        const uint32_t j2 = 2u * j;   // < 8192
+        const float2 w1 = ctw<S>(cmul(tw->h16k[j >> 7], tw->l16k[j & 127]));
+        const float2 w2 = ctw<S>(cmul(tw->h16k[j2 >> 7], tw->l16k[j2 & 127]));
+        v[b + 4] = cmul(v[b + 4], w1);
+        v[b + 8] = cmul(v[b + 8], w2);
+        v[b + 12] = cmul(v[b + 12], cmul(w1, w2));
+        dft4<S>(v[b], v[b + 4], v[b + 8], v[b + 12]);
+    }
+}
+
+}  // namespace sdbe

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "sdbe_device.hpp"
+#include "sdbe_mfma.hpp"
 #include "../../include/sdbe_stage.h"
 
 using namespace sdbe;
@@ -231,7 +232,9 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables
 // Workgroups are dispatched round-robin over the 8 XCDs; with xcd_map the 39
 // rows of a block-pol all run on one XCD (requires the chunk to be a multiple
 // of 8).  The packed spectrum is stored row-major: Zf[k1 + 39 k2] at
-// k1 * 16384 + k2.
+// k1 * 16384 + k2.  P = 0: FP32 VALU FFTs; P = FftMode: radix-16 passes on the
+// matrix cores (fft16k_mfma).
+template <int P>
 __global__ void __launch_bounds__(1024)
 k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int xcd_map) {
     __shared__ float lre[LDS_PAD_N];
@@ -250,14 +253,55 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
     }
     const float2 *src = V + ((size_t)bpl * NBLK + k1) * NCH;
     float2 v[16];
+    if constexpr (P == 0) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = src[t + 1024 * q];
-    fft16k<+1>(v, lre, lim, &ftw);
-    // e^{-2 pi i (t + 1024 q) k1 / M'} = A(t) * B(q); B is uniform over the workgroup
-    const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
+        for (int q = 0; q < 16; ++q) v[q] = src[t + 1024 * q];
+        fft16k<+1>(v, lre, lim, &ftw);
+    }
+    // the MFMA path runs both transforms with the inverse-sign operands: the
+    // forward FFT is conj(IFFT(conj(x)))
+    MfmaW<+1, P> W;
+    if constexpr (P != 0) {
+        W.init();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<-1>(tb.nmp, (uint32_t)(1024 * q * k1))));
-    fft16k<-1>(v, lre, lim, &ftw);
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[4 * g + j] = src[mfma_elem(g, j)];
+        fft16k_mfma<+1, P>(v, lre, lim, &ftw, W);
+    }
+    if constexpr (P == 0) {
+        // e^{-2 pi i (t + 1024 q) k1 / M'} = A(t) * B(q); B is uniform over the workgroup
+        const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<-1>(tb.nmp, (uint32_t)(1024 * q * k1))));
+        fft16k<-1>(v, lre, lim, &ftw);
+    } else {
+        // natural order -> MFMA layout through LDS; the twiddle of element
+        // e = d + 1024 r is applied there: e^{-2 pi i e k1 / M'} = A(d) * B(r)
+        const int rd = t + (t >> 4), kb = (t & 63) >> 4;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            lre[rd + 1088 * q] = v[q].x;
+            lim[rd + 1088 * q] = v[q].y;
+        }
+        __syncthreads();
+        float2 Br[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Br[j] = tw<-1>(tb.nmp, (uint32_t)(1024 * (4 * kb + j) * k1));
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0) & 1023) * k1));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int pp = lds_pos(mfma_elem(g, j));
+                v[4 * g + j] = cconj(cmul(make_float2(lre[pp], lim[pp]), cmul(Ad, Br[j])));
+            }
+        }
+        fft16k_mfma<+1, P>(v, lre, lim, &ftw, W);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = cconj(v[q]);
+    }
     // row-major [k1][k2]: Zf[k1 + 39 k2] lives at k1 * 16384 + k2
     float2 *dst = Zf + ((size_t)bpl * NBLK + k1) * NCH + t;
 #pragma unroll
@@ -540,7 +584,7 @@ struct OutDesc {
 // samples (2t, 2t+1) = (Re, Im) z[t], scaled by 2^-36 to numpy units.  Codes
 // per vdif_inout_gpu_thread.cu:577 (v = x - off; ((|v| >= th) | 2) ^ 3 (v < 0));
 // the 4-bit pair code (re | im << 2) of t goes to stage[bp][tx][ty].
-template <bool F32>
+template <bool F32, int P>
 __global__ void __launch_bounds__(1024)
 k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const BpDesc *__restrict__ bpd,
            uint8_t *__restrict__ stage, const int *__restrict__ slot_col, Tables tb) {
@@ -583,14 +627,35 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
     }
     __syncthreads();
     float2 v[16];
+    if constexpr (P == 0) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
-    // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
-    // times a workgroup-uniform one
-    const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
+        for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
+        // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
+        // times a workgroup-uniform one
+        const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
-    fft16k<+1>(v, lre, lim, &ftw);
+        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
+        fft16k<+1>(v, lre, lim, &ftw);
+    } else {
+        // MFMA layout: element c = d + 1024 r, twiddle e^{2 pi i c tx/2^20} = A(d) B(r)
+        const int l = t & 63, kb = l >> 4;
+        float2 Br[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Br[j] = tw<+1>(tb.np, (uint32_t)(1024 * (4 * kb + j) * tx));
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int e0 = mfma_elem(g, 0);
+            const float2 Ad = tw<+1>(tb.np, (uint32_t)((e0 & 1023) * tx));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int pp = lds_pos(mfma_elem(g, j));
+                v[4 * g + j] = cmul(make_float2(lre[pp], lim[pp]), cmul(Ad, Br[j]));
+            }
+        }
+        MfmaW<+1, P> W;
+        W.init();
+        fft16k_mfma<+1, P>(v, lre, lim, &ftw, W);
+    }
     if (F32) {
         GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * t;
 #pragma unroll
@@ -866,6 +931,22 @@ int build_slot_table(sdbe_handle_t *h) {
 }
 
 // Run stages A..C2 over block-pols [0, n_bp) in chunks.
+template <typename K>
+void launch_c2_kernel(sdbe_handle_t *h, int nb, hipStream_t s, const BpDesc *bpd, K kern) {
+    LAUNCH(h, KID_C2, nb, s, kern, dim3(KHI, nb), dim3(1024), (const float *)h->Bc, (const float2 *)h->thr, bpd,
+           h->stage, (const int *)h->slot_col, h->tb);
+}
+
+template <bool F32>
+void launch_c2(sdbe_handle_t *h, int nb, hipStream_t s, const BpDesc *bpd) {
+    switch (h->cfg.fft_precision) {
+    case FFT_BF16X3: launch_c2_kernel(h, nb, s, bpd, k_stage_c2<F32, FFT_BF16X3>); break;
+    case FFT_BF16: launch_c2_kernel(h, nb, s, bpd, k_stage_c2<F32, FFT_BF16>); break;
+    case FFT_FP16: launch_c2_kernel(h, nb, s, bpd, k_stage_c2<F32, FFT_FP16>); break;
+    default: launch_c2_kernel(h, nb, s, bpd, k_stage_c2<F32, 0>);
+    }
+}
+
 // Run stages A..C2 (+ pack) over the n_bp block-pols described by the device
 // table d_bpd, in chunks.  f32: write float samples instead of codes.
 int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32,
@@ -874,19 +955,35 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
         LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb);
-        LAUNCH(h, KID_B, nb, s, k_stage_b, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
-               (nb % 8) == 0 ? 1 : 0);
+        {
+            const int xm = (nb % 8) == 0 ? 1 : 0;
+            switch (h->cfg.fft_precision) {
+            case FFT_BF16X3:
+                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_BF16X3>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V,
+                       h->Zf, h->tb, xm);
+                break;
+            case FFT_BF16:
+                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_BF16>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf,
+                       h->tb, xm);
+                break;
+            case FFT_FP16:
+                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_FP16>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf,
+                       h->tb, xm);
+                break;
+            default:
+                LAUNCH(h, KID_B, nb, s, k_stage_b<0>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
+                       xm);
+            }
+        }
         const int c1_wgs = (C1_TILES * nb + h->c1_tpw - 1) / h->c1_tpw;
         LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(c1_wgs), dim3(256), (const float2 *)h->Zf, (float *)h->Bc, h->part, h->tb,
                nb, h->c1_tpw);
         LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
         LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(256), (const double2 *)h->ms, nb, bpd, h->latch, h->thr);
         if (f32) {
-            LAUNCH(h, KID_C2, nb, s, k_stage_c2<true>, dim3(KHI, nb), dim3(1024), (const float *)h->Bc,
-                   (const float2 *)h->thr, bpd, h->stage, (const int *)h->slot_col, h->tb);
+            launch_c2<true>(h, nb, s, bpd);
         } else {
-            LAUNCH(h, KID_C2, nb, s, k_stage_c2<false>, dim3(KHI, nb), dim3(1024), (const float *)h->Bc,
-                   (const float2 *)h->thr, bpd, h->stage, (const int *)h->slot_col, h->tb);
+            launch_c2<false>(h, nb, s, bpd);
             LAUNCH(h, KID_PACK, nb, s, k_pack, dim3((NCH / 64) * nb), dim3(256), (const uint8_t *)h->stage, bpd, nb,
                    od);
         }
@@ -1066,6 +1163,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     if (cfg.chunk_blockpols < 2 || cfg.chunk_blockpols > 256 || (cfg.chunk_blockpols & 1)) return SDBE_ERR_ARG;
     if (cfg.max_spectra_blockpols < 1 || cfg.max_batch_blockpols < 0) return SDBE_ERR_ARG;
     if (cfg.c1_tiles_per_wg < 0 || cfg.c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
+    if (cfg.fft_precision < SDBE_FFT_FP32 || cfg.fft_precision > SDBE_FFT_FP16) return SDBE_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SDBE_ERR_NODEV;
     sdbe_handle_t *h = new (std::nothrow) sdbe_handle();

@@ -32,7 +32,7 @@ def station_code(s):
 class SdbeStage:
     def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=256, device=0,
                  station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0), use_graph=False,
-                 c1_tiles_per_wg=0, max_batch=0):
+                 c1_tiles_per_wg=0, max_batch=0, fft_precision=L.FFT_FP32):
         self.lib = L.load()
         cfg = L.Config()
         L.check(self.lib.sdbe_default_config(C.byref(cfg)), "sdbe_default_config")
@@ -48,6 +48,7 @@ class SdbeStage:
         cfg.use_graph = 1 if use_graph else 0
         cfg.c1_tiles_per_wg = c1_tiles_per_wg
         cfg.max_batch_blockpols = max_batch
+        cfg.fft_precision = fft_precision
         self.cfg = cfg
         self.device = device
         self.out_mode = out_mode

@@ -1,0 +1,85 @@
+"""Precision variants of the resampler (matrix-core FFT passes, config 3):
+every mode against the FP64 oracle on the same input -- random 2-bit spectra
+plus a strong tone (0 dB sigma) and a weak tone (-60 dB sigma), the 60 dB
+dynamic-range case.  Reported per block-pol: RMS and max |x - x64| / sigma64,
+and the number of 2-bit codes that differ from the oracle's.
+
+Bars (north star: RMS <= 1e-3 sigma); the max error of every mode is 50-70
+times its RMS (the largest samples, at the decode-bias impulse that starts
+each snapshot, carry the largest absolute error), so the max bars are
+about 100 x the RMS bars:
+* FP32 (VALU):  RMS <= 2e-6 sigma, max <= 5e-5 sigma (the FP32 pipeline);
+* BF16X3:       RMS <= 1e-4 sigma, max <= 2e-3 sigma;
+* FP16:         RMS <= 1e-3 sigma, max <= 0.1 sigma;
+* BF16:         RMS <= 1e-2 sigma, max <= 1 sigma (one bf16 product per pass:
+                reported, outside the north-star bound).
+The weak tone must stand 10x above the error floor at its frequency.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sdbe_oracle as O
+from sdbe_amd import synth as S
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"fp32": (0, 2e-6, 5e-5), "bf16x3": (1, 1e-4, 2e-3), "bf16": (2, 1e-2, 1.0), "fp16": (3, 1e-3, 0.1)}
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def spectra():
+    """2 block-pols: CN(0,1) noise (2-bit), a 0 dB tone at channel 4321 and a
+    -60 dB tone (amplitude 1e-3 sigma) at channel 9876, quantised together."""
+    rng = S.rng_for(3003)
+    codes = S.gaussian_codes(rng, 2 * 39, 1.0, ((4321, 1.0, 0.0113), (9876, 1e-6, 0.0371)))
+    beng = S.codes_to_beng_int8(codes).reshape(2, 39, 16384, 2)
+    x64 = [O.resample_block(O.inflate(beng[b])) for b in range(2)]
+    return beng, x64
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_precision_mode_vs_fp64(torch_cuda, spectra, mode):
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    from sdbe_amd.stage import SdbeStage
+    beng, x64 = spectra
+    prec, rms_bar, max_bar = MODES[mode]
+    st = SdbeStage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD, chunk=2, max_spectra=2,
+                   fft_precision=prec)
+    d = torch.from_numpy(beng).cuda()
+    x, stats = st.resample_f32(d)
+    x = x.cpu().numpy().astype(np.float64)
+    codes = st.process_spectra(d).cpu().numpy().view(np.uint32).reshape(2, -1)
+    for b in range(2):
+        mean, sig = O.block_stats(x64[b])
+        err = x[b] - x64[b]
+        rms = float(np.sqrt(np.mean(err ** 2)) / sig)
+        mx = float(np.abs(err).max() / sig)
+        c64 = O.quantize2bit(x64[b], sig, mean)
+        n_diff = int((O.unpack_codes(codes[b]) != c64).sum())
+        # weak tone: bin 39 * 9876 of the output spectrum vs the error floor there
+        spec_err = np.abs(np.fft.rfft(err))
+        spec_sig = np.abs(np.fft.rfft(x64[b]))
+        kw = 39 * 9876
+        tone_over_floor = float(spec_sig[kw] / (np.median(spec_err[kw - 2000:kw + 2000]) + 1e-300))
+        _say("[prec] %-6s bp %d: RMS %.3e sigma, max %.3e sigma, %d codes differ (of %d), "
+             "weak tone / error floor %.1f" % (mode, b, rms, mx, n_diff, c64.size, tone_over_floor))
+        assert rms <= rms_bar
+        assert mx <= max_bar
+        assert tone_over_floor > 10.0
+    st.close()

@@ -376,9 +376,13 @@ __device__ __forceinline__ void c1_fetch(const float2 *__restrict__ Zf, const Ta
 __global__ void __launch_bounds__(256)
 k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__restrict__ part, Tables tb, int nb,
            int tpw) {
-    __shared__ float2 sz[32][41];
+    // sz (packed-spectrum tile) is dead once Y is formed and sc (first-pass
+    // tile) is written after that barrier, so they share storage: 28 KB of LDS
+    // per workgroup, five workgroups per CU.
+    __shared__ float2 szc[32 * 65];
     __shared__ float2 sy[32][41];
-    __shared__ float2 sc[32][65];
+    float2 (*sz)[41] = reinterpret_cast<float2 (*)[41]>(szc);
+    float2 (*sc)[65] = reinterpret_cast<float2 (*)[65]>(szc);
     __shared__ float2 rowm[40], rowl[64], w64[64];
     __shared__ double red[2][4];
     const int t = threadIdx.x, col = t & 31, rw = t >> 5, pcol = col ^ 16;

@@ -195,6 +195,38 @@ typedef struct vdif_8pac_packet { vdif_in_packet_t pkt1; vtp_plus_pac_t pac7[7];
 SDBE_LAYOUT_ASSERT(sizeof(vdif_8pac_packet_t) == 1056 + 7 * 1064, "vdif_8pac_packet_t");
 void unpack_8pac(vdif_in_packet_t *dest, vdif_8pac_packet_t *src, int n_src);
 
+/* ------------------------------ device side of the input / output threads */
+/* HIP implementations under the reference's link names
+ * (software/aphids/vdif_in_databuf_cuda.cu:47-107,
+ *  vdif_out_databuf_cuda.cu:48-93): pinned host group buffers, device group
+ * buffers, and the asynchronous copies the input thread starts after a group
+ * completes and the output thread starts for every filled output slot.  Each
+ * copy runs on its own stream (default priority), kept in memcpy_stream until
+ * the completion check destroys it.  All return 1 on success, -1 on failure.
+ * The device of slot `index` is index % (number of HIP devices), as the
+ * reference's index % NUM_GPU.                                            */
+int get_bgv_cpu_memory_cuda(beng_group_vdif_buffer_t **bgv_buf_cpu, int index);
+int get_bgv_gpu_memory_cuda(beng_group_vdif_buffer_t **bgv_buf_gpu, int index);
+/* Copies only the packet_count packets the slot holds (the reference copies
+ * the whole 84 MB slot); sets gpu_id and memcpy_stream.                    */
+int transfer_beng_group_to_gpu_cuda(vdif_in_databuf_t *bgc_buf, int index);
+/* 0 while the copy runs, 1 when done (the stream is destroyed then).       */
+int check_transfer_beng_group_to_gpu_complete_cuda(vdif_in_databuf_t *bgc_buf, int index);
+int get_vdg_cpu_memory_cuda(vdif_out_data_group_t **vdg_buf_cpu, int index);
+int transfer_vdif_group_to_cpu_cuda(vdif_out_databuf_t *qs_buf, int index);
+/* Waits for the copy, destroys its stream, returns 1.                      */
+int check_transfer_vdif_group_to_cpu_complete_cuda(vdif_out_databuf_t *qs_buf, int index);
+
+/* The reference's wrappers of the above (vdif_in_databuf.c:169-190,
+ * vdif_out_databuf.c:16-33), for builds without the reference's files.     */
+int get_bgv_cpu_memory(beng_group_vdif_buffer_t **bgv_buf_cpu, int index);
+int get_bgv_gpu_memory(beng_group_vdif_buffer_t **bgv_buf_gpu, int index);
+int transfer_beng_group_to_gpu(vdif_in_databuf_t *bgc_buf, int index);
+int check_transfer_beng_group_to_gpu_complete(vdif_in_databuf_t *bgc_buf, int index);
+int get_vdg_cpu_memory(vdif_out_data_group_t **vdg_buf_cpu, int index);
+int transfer_vdif_group_to_cpu(vdif_out_databuf_t *qs_buf, int index);
+int check_transfer_vdif_group_to_cpu_complete(vdif_out_databuf_t *qs_buf, int index);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,7 +54,7 @@ def build_hashpipe_plugin():
     hashpipe API stand-in (production builds use the real hashpipe headers)."""
     tmp = HASHPIPE_LIB + ".tmp"
     srcs = [os.path.join(CSRC, "vdif_inout_gpu_thread.c"), os.path.join(CSRC, "aphids_host.c"),
-            os.path.join(COMPAT, "hashpipe_compat.c")]
+            os.path.join(CSRC, "aphids_databuf_hip.c"), os.path.join(COMPAT, "hashpipe_compat.c")]
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     # plain C against the HIP runtime API (the HIP headers need the platform macro)
     _run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-D__HIP_PLATFORM_AMD__",

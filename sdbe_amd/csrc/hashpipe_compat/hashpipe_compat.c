@@ -174,6 +174,7 @@ void hashpipe_info(const char *name, const char *msg, ...) {
  * caller fills input blocks through the returned pointers:
  *   hpc_start(name)                -> starts the plugin thread
  *   hpc_in_block(i) / hpc_out_block(j) -> block memory
+ *   hpc_in_databuf() / hpc_out_databuf() -> the two databufs
  *   hpc_set_in_filled(i), hpc_wait_out_filled(j, timeout_ms), hpc_set_out_free(j)
  *   hpc_stop()                     -> run_threads() turns false, thread joined */
 static hashpipe_thread_args_t g_args;
@@ -212,6 +213,8 @@ void *hpc_out_block(int j) {
     return d ? (char *)d + d->header_size + d->block_size * (size_t)j : NULL;
 }
 
+void *hpc_in_databuf(void) { return g_args.ibuf; }
+void *hpc_out_databuf(void) { return g_args.obuf; }
 int hpc_set_in_filled(int i) { return hashpipe_databuf_set_filled(g_args.ibuf, i); }
 int hpc_in_status(int i) { return hashpipe_databuf_block_status(g_args.ibuf, i); }
 int hpc_set_out_free(int j) { return hashpipe_databuf_set_free(g_args.obuf, j); }

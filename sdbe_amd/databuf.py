@@ -110,6 +110,15 @@ def load():
         "unpack_8pac": (None, [C.c_void_p, C.c_void_p, C.c_int]),
         "vdif_in_databuf_create": (C.c_void_p, [C.c_int, C.c_int]),
         "vdif_out_databuf_create": (C.c_void_p, [C.c_int, C.c_int]),
+        "get_bgv_cpu_memory_cuda": (C.c_int, [P(C.c_void_p), C.c_int]),
+        "get_bgv_gpu_memory_cuda": (C.c_int, [P(C.c_void_p), C.c_int]),
+        "transfer_beng_group_to_gpu_cuda": (C.c_int, [C.c_void_p, C.c_int]),
+        "check_transfer_beng_group_to_gpu_complete_cuda": (C.c_int, [C.c_void_p, C.c_int]),
+        "get_vdg_cpu_memory_cuda": (C.c_int, [P(C.c_void_p), C.c_int]),
+        "transfer_vdif_group_to_cpu_cuda": (C.c_int, [C.c_void_p, C.c_int]),
+        "check_transfer_vdif_group_to_cpu_complete_cuda": (C.c_int, [C.c_void_p, C.c_int]),
+        "hpc_in_databuf": (C.c_void_p, []),
+        "hpc_out_databuf": (C.c_void_p, []),
         "hpc_start": (C.c_int, [C.c_char_p]),
         "hpc_in_block": (C.c_void_p, [C.c_int]),
         "hpc_out_block": (C.c_void_p, [C.c_int]),

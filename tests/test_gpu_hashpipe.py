@@ -79,3 +79,90 @@ def test_plugin_two_groups_match_cabi():
         want = st.process_group(dpk[i], pk.shape[0], b0).cpu().numpy().reshape(-1)
         assert np.array_equal(got[i], want), i
     st.close()
+
+
+def test_host_to_host_chain_through_databuf_copies():
+    """The input thread's side (host bookkeeping into pinned group buffers,
+    HIP host-to-device copy), the plugin, and the output thread's side (HIP
+    device-to-host copy into a pinned buffer): the host bytes equal the C-ABI
+    output for the same packets.  Prints the PCIe copy rates."""
+    import time
+
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from sdbe_amd import _lib as L
+    from sdbe_amd import databuf as D
+    from sdbe_amd.stage import SdbeStage
+
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipHostFree.argtypes = [C.c_void_p]
+    hip.hipFree.argtypes = [C.c_void_p]
+    _say("[chain] generating 2 groups")
+    groups, _ = S.make_group_packets(seed=63, n_groups=2, drop_frac=0.001)
+    tmpl = O.fill_header_template(groups[0][0][0, :8], (100, 143_000_000))
+    hp = D.load()
+    assert hp.hpc_start(b"vdif_inout_gpu_thread") == 0
+    pinned, devbufs = [], []
+    try:
+        db = D.VdifInDatabuf.from_address(hp.hpc_in_databuf())
+        b_first = groups[0][1]
+        for i in range(2):
+            cpu, gpu = C.c_void_p(), C.c_void_p()
+            assert hp.get_bgv_cpu_memory_cuda(C.byref(cpu), i) == 1
+            assert hp.get_bgv_gpu_memory_cuda(C.byref(gpu), i) == 1
+            pinned.append(cpu.value)
+            devbufs.append(gpu.value)
+            hp.init_beng_group(C.byref(db.bgc[i]), cpu, gpu, b_first + 39 * i)
+        _say("[chain] inserting packets through the group window")
+        for i, (pk, b0) in enumerate(groups):
+            assert b0 == b_first + 39 * i
+            for row in pk:
+                p = D.VdifInPacket.from_buffer(row)
+                off = hp.get_beng_group_index_offset(C.byref(db), 0, C.byref(p))
+                assert off == i
+                hp.insert_vdif_in_beng_group_buffer(C.byref(db), 0, off, C.byref(p))
+            assert db.bgc[i].beng_group_vdif_packet_count == pk.shape[0]
+        h2d = []
+        for i in range(2):
+            C.memmove(C.addressof(db.bgc[i].vdif_header_template), tmpl.ctypes.data, 32)
+            if i:
+                db.bgc[i].vdif_header_template.w0.invalid = 1
+            t0 = time.perf_counter()
+            assert hp.transfer_beng_group_to_gpu_cuda(C.byref(db), i) == 1
+            while True:
+                rv = hp.check_transfer_beng_group_to_gpu_complete_cuda(C.byref(db), i)
+                assert rv in (0, 1)
+                if rv == 1:
+                    break
+            h2d.append((time.perf_counter() - t0, groups[i][0].nbytes))
+            assert db.bgc[i].gpu_id == 0
+            assert hp.hpc_set_in_filled(i) == 0
+        got, d2h = [], []
+        for j in range(2):
+            _say("[chain] output slot %d" % j)
+            assert hp.hpc_wait_out_filled(j, 120000) == 0
+            qs = D.QuantizedStorage.from_address(hp.hpc_out_block(j))
+            host = C.c_void_p()
+            assert hp.get_vdg_cpu_memory_cuda(C.byref(host), j) == 1
+            pinned.append(host.value)
+            qs.vdg_buf_cpu = host.value
+            t0 = time.perf_counter()
+            assert hp.transfer_vdif_group_to_cpu_cuda(C.c_void_p(hp.hpc_out_databuf()), j) == 1
+            assert hp.check_transfer_vdif_group_to_cpu_complete_cuda(C.c_void_p(hp.hpc_out_databuf()), j) == 1
+            d2h.append(time.perf_counter() - t0)
+            got.append(np.ctypeslib.as_array((C.c_uint8 * (2 * 8192 * 8192)).from_address(host.value)).copy())
+        for i, (dt, nb) in enumerate(h2d):
+            _say("[chain] group %d: H2D %.1f MB in %.2f ms (%.1f GB/s), D2H 134.2 MB in %.2f ms (%.1f GB/s)"
+                 % (i, nb / 1e6, dt * 1e3, nb / dt / 1e9, d2h[i] * 1e3, 2 * 8192 * 8192 / d2h[i] / 1e9))
+    finally:
+        hp.hpc_stop()
+    st = SdbeStage(policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD)
+    for i, (pk, b0) in enumerate(groups):
+        want = st.process_group(st.to_device_packets(pk), pk.shape[0], b0).cpu().numpy().reshape(-1)
+        assert np.array_equal(got[i], want), i
+    st.close()
+    for ptr in pinned:
+        hip.hipHostFree(ptr)
+    for ptr in devbufs:
+        hip.hipFree(ptr)

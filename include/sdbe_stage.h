@@ -135,7 +135,7 @@ typedef struct sdbe_stats {
     int32_t n_unique;           /* (frame, cid, fid) slots filled               */
     int32_t n_missing;          /* slots zero-filled                            */
     int32_t n_blockpols;        /* block-pols resampled                         */
-    int32_t reserved;
+    int32_t n_frames_per_pol;   /* frames written per polarisation              */
     double  latched_threshold;  /* REFERENCE policy latch (0 if not latched)    */
     double  latched_offset;
 } sdbe_stats_t;
@@ -237,6 +237,20 @@ typedef struct sdbe_stream_desc {
  * iteration (vdif_inout_gpu_thread.cu:923-1008).                            */
 int sdbe_process_blocks(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int32_t n_streams,
                         void *stream);
+
+/* Re-framed group output: the output thread's time alignment done on the
+ * device (software/aphids/vdif_out_net_thread.c:203-290).  With enable = 1 the
+ * next sdbe_process_group starts an aligned stream: its frames begin
+ * skip_int32 + 2048 skip_frames words into the group's per-polarisation
+ * output (the values sdbe_stream_state_from_template returns), so it writes
+ * 8192 - ceil((skip_int32 + 2048 skip_frames) / 2048) frames per
+ * polarisation; every later group writes 8192 frames, the first of which
+ * opens with the words the previous group could not fill a frame with.  The
+ * frames written per polarisation are in sdbe_stats_t.n_frames_per_pol, and
+ * FRAMES-mode headers are stamped over exactly those frames from the stream
+ * state.  enable = 0 returns to group-aligned output.  No reference
+ * counterpart as a call: the reference does this on the host after the copy. */
+int sdbe_set_output_alignment(sdbe_handle_t *h, int32_t enable, int32_t skip_int32, int32_t skip_frames);
 
 /* Clear REFERENCE latch slot `slot` (asynchronously, on `stream`).          */
 int sdbe_reset_latch(sdbe_handle_t *h, int32_t slot, void *stream);

@@ -829,3 +829,60 @@ def pfb_inverse(spec, window, lblock=PFB_LBLOCK):
     for p in range(ntap):
         x[p:p + nblock] += w[p] * z
     return x.reshape(-1)
+
+
+class OutputAligner:
+    """The output thread's re-framing of the stage's per-group payload stream
+    (vdif_out_net_thread.c:198-317): on a group whose header template is
+    valid, derive secs / df / psn and the sample skip (output_alignment); the
+    frames sent are the first num_usable frames of [leftover of the previous
+    group] + [this group's words from skip_int32 + 2048 skip_frames on]; the
+    last leftover_int32 words of the group are kept for the next one, and
+    leftover_int32 keeps the value of the last valid template.  Frames are
+    stamped as the output thread does (df wraps at 125000 with secs + 1,
+    psn + 1, pol = channel)."""
+
+    WPF = OUT_PAYLOAD_BYTES // 4         # 2048 words per frame
+
+    def __init__(self, station=station_from_str("XX"), bdc=0, rx=0, bit_depth=2):
+        self.leftover = [None, None]
+        self.leftover_int32 = 0
+        self.station, self.bdc, self.rx, self.bit_depth = station, bdc, rx, bit_depth
+        self.secs = self.df = self.ref_epoch = 0
+        self.psn = 1
+
+    def push(self, payload, template_words):
+        """payload: uint32 [2][N] (the group's words per polarisation).
+        Returns (frames uint32 [2][n][2048], headers uint32 [2][n][8])."""
+        N = payload.shape[1]
+        skip_int32 = skip_frames = 0
+        num_usable = N // self.WPF
+        f = in_header_fields(np.asarray(template_words, dtype=np.uint32))
+        if not int(f["invalid"]):
+            a = output_alignment(template_words, N, self.bit_depth)
+            self.secs, self.df, self.ref_epoch, self.psn = a.secs, a.df, a.ref_epoch, 1
+            skip_int32, skip_frames, num_usable = a.skip_int32, a.skip_frames, a.num_usable
+            self.leftover_int32 = a.leftover_int32
+        L = self.leftover_int32
+        frames = np.zeros((2, num_usable, self.WPF), dtype=np.uint32)
+        for pol in (0, 1):
+            parts = []
+            if self.leftover[pol] is not None:
+                parts.append(self.leftover[pol][:L])
+            start = skip_int32 + skip_frames * self.WPF
+            parts.append(payload[pol][start:start + N - skip_int32 - L])
+            stream = np.concatenate(parts)
+            frames[pol] = stream[:num_usable * self.WPF].reshape(num_usable, self.WPF)
+            if L > 0:
+                self.leftover[pol] = payload[pol][N - L:].copy()
+        headers = np.zeros((2, num_usable, 8), dtype=np.uint32)
+        for ii in range(num_usable):
+            for pol in (0, 1):
+                headers[pol, ii] = out_header_words(self.secs, self.df, self.ref_epoch, self.station, pol,
+                                                    self.bdc, self.rx, self.psn, bps=self.bit_depth - 1)
+            self.df += 1
+            if self.df == VDIF_OUT_FRAMES_PER_SECOND:
+                self.df = 0
+                self.secs += 1
+            self.psn += 1
+        return frames, headers

@@ -99,7 +99,7 @@ class Stats(C.Structure):
         ("n_unique", C.c_int32),
         ("n_missing", C.c_int32),
         ("n_blockpols", C.c_int32),
-        ("reserved", C.c_int32),
+        ("n_frames_per_pol", C.c_int32),
         ("latched_threshold", C.c_double),
         ("latched_offset", C.c_double),
     ]
@@ -140,6 +140,7 @@ PROTOTYPES = [
      [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sdbe_process_blocks", C.c_int, [C.c_void_p, C.POINTER(StreamDesc), C.c_int32, C.c_void_p]),
     ("sdbe_reset_latch", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    ("sdbe_set_output_alignment", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     ("sdbe_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     # include/sdbe_pfb.h
     ("sdbe_pfb_create", C.c_int, [C.c_int, C.c_int32, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),

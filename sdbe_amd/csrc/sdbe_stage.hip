@@ -173,7 +173,7 @@ struct BpDesc {
     uint32_t w4;             // pol | bdc << 1 | rx << 2 | (edv5 = 2) << 24
     int16_t flip, pol, policy, slot;
     float fixed_th, fixed_off;
-    uint32_t pad;
+    int32_t blk;             // block index within its group (re-framed group output)
 };
 static_assert(sizeof(BpDesc) == 64, "BpDesc layout");
 
@@ -581,6 +581,15 @@ __global__ void __launch_bounds__(256) k_thresh(const double2 *__restrict__ ms, 
 struct OutDesc {
     int32_t frame_bytes;     // 8192 (PAYLOAD) or 8224 (FRAMES)
     int32_t hdr_bytes;       // 0 or 32
+    int32_t bp_headers;      // stamp 64 frames per block-pol (k_headers)
+    int32_t reserved;
+    // re-framed group output (output-thread time alignment): block blk's
+    // stream byte q goes to group stream byte blk * 524288 + q + shift_bytes;
+    // bytes below 0 are dropped, bytes at or beyond lim_bytes go to
+    // lead[pol][. - lim_bytes], the head of the next group's first frame
+    int64_t shift_bytes;
+    int64_t lim_bytes;
+    uint8_t *lead;           // [2][8192]
 };
 
 // Stage C2: workgroup = (tx, block-pol).
@@ -701,6 +710,8 @@ k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb
     __syncthreads();
     // 64 ty x 32 bytes = 2048 contiguous stream bytes starting at 32 * ty0
     GAS uint8_t *ob = gptr(bpd[bpl].out) + od.hdr_bytes;
+    const int64_t blk_pos = (int64_t)bpd[bpl].blk * (OUT_FRAMES_BLOCK * OUT_FRAME_PAYLOAD);
+    const int pol = bpd[bpl].pol;
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
         const int e = it * 256 + t;            // 4-byte word index within the 2048 bytes
@@ -711,9 +722,15 @@ k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb
             const int i = i0 + b;
             wv |= (uint32_t)(tile[ty][2 * i] | (tile[ty][2 * i + 1] << 4)) << (8 * b);
         }
-        const int qb = 32 * (ty0 + ty) + i0;    // stream byte
-        const int f = qb >> 13, off = qb & (OUT_FRAME_PAYLOAD - 1);
-        *reinterpret_cast<GAS uint32_t *>(ob + (int64_t)f * od.frame_bytes + off) = wv;
+        const int64_t qb = 32 * (ty0 + ty) + i0 + od.shift_bytes;   // stream byte, relative to the block
+        const int64_t gpos = blk_pos + qb;                          // ... and to the group
+        if (gpos < 0) continue;
+        if (gpos >= od.lim_bytes) {
+            *reinterpret_cast<GAS uint32_t *>(gptr(od.lead) + pol * OUT_FRAME_PAYLOAD + (gpos - od.lim_bytes)) = wv;
+            continue;
+        }
+        const int64_t f = qb >> 13, off = qb & (OUT_FRAME_PAYLOAD - 1);   // f may be negative: frames of earlier blocks
+        *reinterpret_cast<GAS uint32_t *>(ob + f * od.frame_bytes + off) = wv;
     }
 }
 
@@ -734,6 +751,40 @@ __global__ void __launch_bounds__(256) k_headers(const BpDesc *__restrict__ bpd,
     GAS u32x4 *dst = reinterpret_cast<GAS u32x4 *>(gptr(d.out) + (int64_t)n * od.frame_bytes);
     dst[0] = u32x4{secs & 0x3FFFFFFFu, (df & 0xFFFFFFu) | d.w1hi, (uint32_t)((OUT_FRAME_PAYLOAD + 32) / 8), d.w3};
     dst[1] = u32x4{d.w4, 0u, (uint32_t)(psn & 0xFFFFFFFFu), (uint32_t)(psn >> 32)};
+}
+
+// Headers of n frames per polarisation of a re-framed group: frame n of pol p
+// at base + p * pol_stride + n * frame_bytes, stamped from (secs, df, psn).
+struct HdrLin {
+    uint8_t *base;
+    int64_t pol_stride;
+    int32_t frame_bytes, n_frames;
+    uint32_t secs, df, w1hi, w3, w4;
+    uint64_t psn;
+};
+__global__ void __launch_bounds__(256) k_headers_lin(HdrLin hd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * hd.n_frames) return;
+    const int pol = i / hd.n_frames, n = i - pol * hd.n_frames;
+    const uint64_t dft = (uint64_t)hd.df + (uint64_t)n;
+    const uint32_t df = (uint32_t)(dft % 125000u);
+    const uint32_t secs = hd.secs + (uint32_t)(dft / 125000u);
+    const uint64_t psn = hd.psn + (uint64_t)n;
+    GAS u32x4 *dst = reinterpret_cast<GAS u32x4 *>(gptr(hd.base) + (int64_t)pol * hd.pol_stride +
+                                                    (int64_t)n * hd.frame_bytes);
+    dst[0] = u32x4{secs & 0x3FFFFFFFu, (df & 0xFFFFFFu) | hd.w1hi, (uint32_t)((OUT_FRAME_PAYLOAD + 32) / 8), hd.w3};
+    dst[1] = u32x4{hd.w4 | (uint32_t)pol, 0u, (uint32_t)(psn & 0xFFFFFFFFu), (uint32_t)(psn >> 32)};
+}
+
+// The previous group's tail (n_bytes per polarisation) becomes the head of
+// this group's first frame.
+__global__ void __launch_bounds__(256) k_lead_copy(const uint8_t *__restrict__ lead, uint8_t *base, int64_t pol_stride,
+                                                   int hdr_bytes, int n_bytes) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const int pol = blockIdx.y;
+    if (i >= n_bytes) return;
+    *reinterpret_cast<GAS uint32_t *>(gptr(base) + pol * pol_stride + hdr_bytes + i) =
+        *reinterpret_cast<const GAS uint32_t *>(gptr(lead) + pol * OUT_FRAME_PAYLOAD + i);
 }
 
 }  // namespace
@@ -792,6 +843,13 @@ struct sdbe_handle {
     hipEvent_t g_done = nullptr;
     int g_used = 0;
     int c1_tpw = 8;
+    // re-framed group output (output-thread time alignment)
+    int align_pending = 0;           // the next group starts an aligned stream
+    int align_active = 0;
+    int64_t align_words = 0;         // skip_int32 + 2048 skip_frames of the next aligned start
+    int lead_words = 0;              // words carried from one group into the next
+    uint8_t *lead = nullptr;         // [2][8192] device
+    int n_frames_last = 0;
     // pinned host mirrors
     int *h_cnt = nullptr;
     float *h_latch = nullptr;
@@ -993,7 +1051,7 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
         }
         HIP_TRY(h, hipGetLastError());
     }
-    if (!f32 && od.hdr_bytes) {
+    if (!f32 && od.hdr_bytes && od.bp_headers) {
         LAUNCH(h, KID_HEADERS, n_bp, s, k_headers, dim3((n_bp * OUT_FRAMES_BLOCK + 255) / 256), dim3(256), d_bpd,
                n_bp, od);
         HIP_TRY(h, hipGetLastError());
@@ -1003,8 +1061,13 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
 
 OutDesc out_desc(const sdbe_handle_t *h) {
     OutDesc od;
+    memset(&od, 0, sizeof(od));
     od.hdr_bytes = h->cfg.out_mode == SDBE_OUT_FRAMES ? 32 : 0;
     od.frame_bytes = OUT_FRAME_PAYLOAD + od.hdr_bytes;
+    od.bp_headers = 1;
+    od.shift_bytes = 0;
+    od.lim_bytes = INT64_MAX;
+    od.lead = nullptr;
     return od;
 }
 
@@ -1145,6 +1208,7 @@ void sdbe_destroy(sdbe_handle_t *h) {
     }
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     (void)hipFree(h->g_d);
+    (void)hipFree(h->lead);
     (void)hipHostFree(h->g_h);
     if (h->g_done) (void)hipEventDestroy(h->g_done);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
@@ -1204,6 +1268,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->latch, 2 * SDBE_LATCH_SLOTS * sizeof(float));
     for (int i = 0; i < sdbe_handle::RING; ++i) ALLOC(h->d_bpd[i], (size_t)h->cap_bp * sizeof(BpDesc));
     ALLOC(h->g_d, (size_t)h->cap_bp * sizeof(BpDesc));
+    ALLOC(h->lead, 2 * OUT_FRAME_PAYLOAD);
     ALLOC(h->V, C * NBLK * NCH * sizeof(float2));
     ALLOC(h->Zf, C * MP * sizeof(float2));
     ALLOC(h->Bc, C * KHI * BC_ROW * sizeof(float2));
@@ -1312,7 +1377,7 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
                        int64_t out_pol_stride_bytes, uint8_t *d_bitmap, void *stream) {
     if (!h || !d_out || (n_pkts > 0 && !d_pkts)) return SDBE_ERR_ARG;
     if (n_pkts < 0 || n_pkts > MAP_N) return SDBE_ERR_CAPACITY;
-    const OutDesc od = out_desc(h);
+    OutDesc od = out_desc(h);
     if (out_pol_stride_bytes < (int64_t)SDBE_OUT_FRAMES_PER_GROUP * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
@@ -1330,12 +1395,57 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
                        (int64_t)blk * OUT_FRAMES_BLOCK * od.frame_bytes;
         set_bp(hb[bp], rows, out, state_plus(h->st, (int64_t)blk * OUT_FRAMES_BLOCK), pol, 0, c.thresh_policy, 0,
                c.fixed_threshold, c.fixed_offset, c.station_id, c.bdc_sideband, c.rx_sideband);
+        hb[bp].blk = blk;
     }
     if ((rc = ring_upload(h, slot, n_bp, s))) return rc;
+    // Re-framed output: the stream of 2048-word frames starts align_words
+    // into the first aligned group; a group's last lead_words words open the
+    // next group's first frame (the output thread's time alignment,
+    // vdif_out_net_thread.c:203-290, done before the copy to the host).
+    int n_out = SDBE_OUT_FRAMES_PER_GROUP;
+    if (h->align_pending || h->align_active) {
+        int64_t sigma;
+        if (h->align_pending) {
+            const int64_t fr = (h->align_words + 2047) / 2048;
+            n_out = SDBE_OUT_FRAMES_PER_GROUP - (int)fr;
+            h->lead_words = (int)(fr * 2048 - h->align_words);
+            sigma = -h->align_words;
+            h->align_pending = 0;
+            h->align_active = 1;
+        } else {
+            sigma = h->lead_words;
+            if (h->lead_words > 0) {
+                hipLaunchKernelGGL(k_lead_copy, dim3((h->lead_words + 255) / 256, 2), dim3(256), 0, s,
+                                   (const uint8_t *)h->lead, (uint8_t *)d_out, out_pol_stride_bytes, od.hdr_bytes,
+                                   4 * h->lead_words);
+                HIP_TRY(h, hipGetLastError());
+            }
+        }
+        od.shift_bytes = 4 * sigma;
+        od.lim_bytes = (int64_t)n_out * OUT_FRAME_PAYLOAD;
+        od.lead = h->lead;
+        od.bp_headers = 0;
+    }
     rc = run_resampler(h, h->d_bpd[slot], n_bp, od, false, nullptr, s);
     if (rc) return rc;
     if ((rc = ring_release(h, slot, s))) return rc;
-    advance_state(h->st, SDBE_OUT_FRAMES_PER_GROUP);
+    if (!od.bp_headers && od.hdr_bytes) {
+        HdrLin hd;
+        hd.base = (uint8_t *)d_out;
+        hd.pol_stride = out_pol_stride_bytes;
+        hd.frame_bytes = od.frame_bytes;
+        hd.n_frames = n_out;
+        hd.secs = h->st.secs;
+        hd.df = h->st.df;
+        hd.psn = h->st.psn;
+        hd.w1hi = (h->st.ref_epoch & 0x3Fu) << 24;
+        hd.w3 = (uint32_t)c.station_id | (1u << 26);
+        hd.w4 = ((uint32_t)(c.bdc_sideband & 1u) << 1) | ((uint32_t)(c.rx_sideband & 1u) << 2) | (2u << 24);
+        LAUNCH(h, KID_HEADERS, 0, s, k_headers_lin, dim3((2 * n_out + 255) / 256), dim3(256), hd);
+        HIP_TRY(h, hipGetLastError());
+    }
+    h->n_frames_last = n_out;
+    advance_state(h->st, n_out);
     h->n_packets_last = n_pkts;
     h->n_bp_last = n_bp;
     h->have_group = 1;
@@ -1373,6 +1483,7 @@ int sdbe_process_spectra(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp, v
                           nullptr, s);
     if (rc) return rc;
     advance_state(h->st, (int64_t)n_bp * OUT_FRAMES_BLOCK);
+    h->n_frames_last = n_bp * OUT_FRAMES_BLOCK;
     h->n_packets_last = 0;
     h->n_bp_last = n_bp;
     h->have_group = 0;
@@ -1502,6 +1613,22 @@ int sdbe_process_blocks(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int
     return finish_call(h, s);
 }
 
+int sdbe_set_output_alignment(sdbe_handle_t *h, int32_t enable, int32_t skip_int32, int32_t skip_frames) {
+    if (!h) return SDBE_ERR_ARG;
+    if (!enable) {
+        h->align_pending = h->align_active = 0;
+        h->lead_words = 0;
+        return SDBE_OK;
+    }
+    const int64_t words = (int64_t)skip_int32 + 2048 * (int64_t)skip_frames;
+    if (skip_int32 < 0 || skip_frames < 0 || words >= (int64_t)(SDBE_OUT_FRAMES_PER_GROUP - 1) * 2048)
+        return SDBE_ERR_ARG;
+    h->align_words = words;
+    h->align_pending = 1;
+    h->align_active = 0;
+    return SDBE_OK;
+}
+
 int sdbe_reset_latch(sdbe_handle_t *h, int32_t slot, void *stream) {
     if (!h || slot < 0 || slot >= SDBE_LATCH_SLOTS) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
@@ -1522,6 +1649,7 @@ int sdbe_get_stats(sdbe_handle_t *h, sdbe_stats_t *o) {
     o->n_duplicate = h->h_cnt[CNT_ACCEPTED] - h->h_cnt[CNT_UNIQUE];
     o->n_missing = h->have_group ? MAP_N - h->h_cnt[CNT_UNIQUE] : 0;
     o->n_blockpols = h->n_bp_last;
+    o->n_frames_per_pol = h->n_frames_last;
     o->latched_threshold = h->h_latch[0];
     o->latched_offset = h->h_latch[1];
     return SDBE_OK;

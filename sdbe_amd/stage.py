@@ -83,6 +83,10 @@ class SdbeStage:
         L.check(self.lib.sdbe_get_stats(self.h, C.byref(s)), "sdbe_get_stats", self.h)
         return {k: getattr(s, k) for k, _ in L.Stats._fields_ if k != "reserved"}
 
+    def set_output_alignment(self, enable=True, skip_int32=0, skip_frames=0):
+        L.check(self.lib.sdbe_set_output_alignment(self.h, 1 if enable else 0, skip_int32, skip_frames),
+                "sdbe_set_output_alignment", self.h)
+
     def reset_latch(self, slot, stream=None):
         L.check(self.lib.sdbe_reset_latch(self.h, slot, _stream(stream)), "sdbe_reset_latch", self.h)
 

@@ -273,3 +273,41 @@ def test_process_group_frames_headers(torch_cuda, one_group):
     assert np.array_equal(out[:, :, 32:], o2)
     st.close()
     st2.close()
+
+
+def test_reframed_group_output_matches_output_thread(torch_cuda):
+    """Device-side time alignment over 3 groups (first template valid, with a
+    sample skip that is not a whole frame and a skip of frames to the next
+    second) equals the output thread's re-framing of the group-aligned output:
+    payloads, headers and frame counts."""
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    from sdbe_amd.stage import state_from_template
+    _say("[align] generating 3 groups")
+    groups, _ = S.make_group_packets(seed=52, n_groups=3, drop_frac=0.001)
+    tmpl = O.fill_header_template(groups[0][0][0, :8], (100, 285_990_000))
+    tmpl[1] = (tmpl[1] & ~np.uint32(0xFFFFFF)) | np.uint32(124_995)          # near the end of a second
+    tmpl[6] = np.uint32(20_000_000)                                         # ps to skip
+    inv = tmpl.copy()
+    inv[0] |= np.uint32(1 << 31)
+    ref = _stage(policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD)
+    st = _stage(policy=L.THRESH_REFERENCE, out_mode=L.OUT_FRAMES, station="Sm")
+    info = state_from_template(tmpl)
+    assert info["skip_frames"] > 0 and info["skip_int32"] % 2048 != 0
+    st.set_state(secs=info["secs"], df=info["df"], ref_epoch=info["ref_epoch"], psn=info["psn"])
+    st.set_output_alignment(True, info["skip_int32"], info["skip_frames"])
+    al = O.OutputAligner(station=O.station_from_str("Sm"))
+    for g, (pk, b0) in enumerate(groups):
+        _say("[align] group %d" % g)
+        d = st.to_device_packets(pk)
+        want_pay = ref.process_group(d, pk.shape[0], b0).cpu().numpy().view(np.uint32).reshape(2, -1)
+        frames, headers = al.push(want_pay, tmpl if g == 0 else inv)
+        out = st.process_group(d, pk.shape[0], b0).cpu().numpy()
+        n = st.stats()["n_frames_per_pol"]
+        assert n == frames.shape[1], (g, n, frames.shape)
+        got_hdr = out[:, :n, :32].copy().view(np.uint32).reshape(2, n, 8)
+        got_pay = out[:, :n, 32:].copy().view(np.uint32).reshape(2, n, 2048)
+        assert np.array_equal(got_hdr, headers), g
+        assert np.array_equal(got_pay, frames), g
+    ref.close()
+    st.close()

@@ -239,3 +239,36 @@ def test_cabi_state_from_template_matches_oracle():
         got = state_from_template(w)
         assert got == dict(secs=a.secs, df=a.df, ref_epoch=a.ref_epoch, psn=1, skip_int32=a.skip_int32,
                            skip_frames=a.skip_frames)
+
+
+@pytest.mark.parametrize("df0,psn_ps", [(124990, 12_345_678), (5, 7), (124000, 300_000)])
+def test_output_aligner_sends_the_stream_from_the_skip(df0, psn_ps):
+    """The frames the output thread sends, concatenated over groups, are the
+    stage's word stream from skip_int32 + 2048 skip_frames on, and their
+    headers count on without a gap (df wrap with secs + 1, psn + 1)."""
+    rng = np.random.default_rng(df0)
+    N = 64 * 2048                                  # a short group keeps the test fast
+    tm = np.zeros(8, dtype=np.uint32)
+    tm[0] = 100
+    tm[1] = (30 << 24) | df0
+    tm[6] = psn_ps & 0xFFFFFFFF
+    tm[7] = psn_ps >> 32
+    a = O.output_alignment(tm, N)
+    inv = tm.copy()
+    inv[0] |= 1 << 31
+    al = O.OutputAligner(station=O.station_from_str("Sm"))
+    pays = [rng.integers(0, 2 ** 32, (2, N), dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    outs = [al.push(pays[0], tm), al.push(pays[1], inv), al.push(pays[2], inv)]
+    D = a.skip_int32 + 2048 * a.skip_frames
+    assert outs[0][0].shape[1] == a.num_usable and outs[1][0].shape[1] == N // 2048
+    for pol in (0, 1):
+        stream = np.concatenate([p[pol] for p in pays])[D:]
+        sent = np.concatenate([o[0][pol].reshape(-1) for o in outs])
+        assert np.array_equal(stream[:sent.size], sent)
+    hdr = np.concatenate([o[1][0] for o in outs])
+    f = O.in_header_fields(hdr)
+    t = f["secs_inre"].astype(np.int64) * 125000 + f["df_num_insec"].astype(np.int64)
+    assert np.all(np.diff(t) == 1)
+    psn = hdr[:, 6].astype(np.uint64) | (hdr[:, 7].astype(np.uint64) << np.uint64(32))
+    assert psn[0] == 1 and np.all(np.diff(psn.astype(np.int64)) == 1)
+    assert (int(f["secs_inre"][0]), int(f["df_num_insec"][0])) == (a.secs, a.df)

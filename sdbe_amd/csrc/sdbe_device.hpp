@@ -16,7 +16,28 @@ constexpr int MP = NBLK * NCH;        // 638,976: complex length of the packed 1
 constexpr int P20 = 1 << 20;          // complex length of the packed 2,097,152 C2R
 constexpr int KHI = 64;               // P20 = KHI * NCH
 constexpr int FFT_THREADS = 1024;     // threads of the 16384-point FFT workgroups
-constexpr int LDS_PAD_N = NCH + NCH / 16;   // padded LDS row (one float per 16)
+constexpr int LDS_FFT_N = 2 * NCH;     // LDS floats of the work array of the 16384-point FFT
+
+// LDS layout of a 16384-point complex work array (one float array of
+// LDS_FFT_N words): rows of 64 elements, the real parts of row i in words
+// 128 i .. 128 i + 63 and the imaginary parts in the next 64 words, so both
+// parts of an element share one address (the imaginary part at +256 bytes,
+// within a DS instruction's offset field).  Within its row an element sits at
+// an XOR swizzle of its index,
+//   lds_swz(e) = e ^ ((e >> 5) & 15) ^ ((e >> 4) & 48),
+// which spreads every access pattern of the FFT passes (reads of t + 1024 r,
+// writes of 16 t + r, 256 (t>>4) + (t&15) + 16 r and 4096 (t>>8) + (t&255) + 256 r)
+// over 32 distinct banks per half-wave and 64 per wave, so none of them has a
+// bank conflict.  lds_re(e) is the word of the real part; it is linear over
+// GF(2) (lds_re(a ^ b) = lds_re(a) ^ lds_re(b)) and bits 10..13 of e pass
+// through: lds_re(e + 1024 r) = lds_re(e) + 2048 r for e < 1024.
+__host__ __device__ constexpr int lds_swz(int e) { return e ^ ((e >> 5) & 15) ^ ((e >> 4) & 48); }
+__host__ __device__ constexpr int lds_re(int e) { return (lds_swz(e) & 63) | ((e & ~63) << 1); }
+__device__ __forceinline__ void lds_put(float *lds, int p, float2 x) {
+    lds[p] = x.x;
+    lds[p + 64] = x.y;
+}
+__device__ __forceinline__ float2 lds_get(const float *lds, int p) { return make_float2(lds[p], lds[p + 64]); }
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -271,54 +292,69 @@ __device__ __forceinline__ void twiddle16_pow(float2 *v, float2 w1, float2 w2, f
 // -------------------------------------------------- 16384-point Stockham FFT
 // 1024 threads; thread t holds elements t + 1024*q (q = 0..15) on entry and on
 // exit (natural order both ways).  Passes: radix 16 (Ns = 1, 16, 256) then
-// radix 4 (Ns = 4096).  LDS holds real and imaginary parts in two padded
-// float arrays of LDS_PAD_N entries each.
-// Padded LDS position of element i is i + (i >> 4).  Every access below is a
-// per-thread base plus a compile-time constant, so no index arithmetic runs
-// per element:
-//   reads  of t + 1024 r                    -> (t + (t >> 4)) + 1088 r
-//   pass-1 writes of 16 t + r               -> 17 t + r
-//   pass-2 writes of (t>>4) 256 + k + 16 r  -> (t>>4) 272 + k + 17 r      (k = t & 15)
-//   pass-3 writes of (t>>8) 4096 + k + 256 r -> (t>>8) 4352 + k + (k>>4) + 272 r  (k = t & 255)
+// radix 4 (Ns = 4096), through the LDS work array (layout of lds_re).  By the
+// linearity of lds_re every access is a per-thread base XOR a compile-time
+// constant:
+//   reads  of t + 1024 r                     -> lds_re(t) + 2048 r
+//   pass-1 writes of 16 t + r                -> lds_re(16 t) ^ r
+//   pass-2 writes of (t>>4) 256 + k + 16 r   -> lds_re((t>>4) 256 + k) ^ lds_re(16 r)   (k = t & 15)
+//   pass-3 writes of (t>>8) 4096 + k + 256 r -> (lds_re((t>>8) 4096 + k) ^ (lds_re(256 r) & 63)) + 512 r
+//                                               (k = t & 255; the base has bits 9..12 clear)
+// Each pass computes its write addresses from an opaque copy of its base
+// (lds_opaque), so the compiler does not keep sixteen addresses of every pass
+// live across the whole transform.
+__device__ __forceinline__ int lds_opaque(int x) {
+    asm("" : "+v"(x));   // not volatile: a volatile asm counts as a memory clobber and
+                         // would keep later uniform loads off the scalar unit
+    return x;
+}
+// fft16k = fft16k_p123 (passes 1-3, ending with the pass-4 inputs read from
+// LDS into v: the LDS array is free after the next barrier) + fft16k_p4
+// (the radix-4 pass, registers only).  Callers may issue work that needs no
+// LDS between the two.
 template <int S>
-__device__ __forceinline__ void fft16k(float2 *v, float *lre, float *lim, const FftTw *tw) {
+__device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *tw);
+template <int S>
+__device__ __forceinline__ void fft16k_p4(float2 *v, const FftTw *tw);
+
+template <int S>
+__device__ __forceinline__ void fft16k(float2 *v, float *lds, const FftTw *tw) {
+    fft16k_p123<S>(v, lds, tw);
+    fft16k_p4<S>(v, tw);
+}
+
+template <int S>
+__device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *tw) {
     const int t = threadIdx.x;
-    const int rd = t + (t >> 4);
+    const int rd = lds_re(t);
     // pass 1: radix 16, Ns = 1 (no twiddle)
     dft16<S>(v);
-    __syncthreads();   // previous users of the LDS arrays are done
+    __syncthreads();   // previous users of the LDS array are done
     {
-        float *wr = lre + 17 * t, *wi = lim + 17 * t;
+        const int base = lds_opaque(lds_re(16 * t));
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            wr[r] = v[r].x;
-            wi[r] = v[r].y;
-        }
+        for (int r = 0; r < 16; ++r) lds_put(lds, base ^ r, v[r]);
     }
     __syncthreads();
     // pass 2: radix 16, Ns = 16
     {
         const int k = t & 15;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = make_float2(lre[rd + 1088 * r], lim[rd + 1088 * r]);
+        for (int r = 0; r < 16; ++r) v[r] = lds_get(lds, rd + 2048 * r);
         twiddle16_pow(v, ctw<S>(tw->t256[k]), ctw<S>(tw->t256[2 * k]), ctw<S>(tw->t256[4 * k]),
                       ctw<S>(tw->t256[8 * k]));
         dft16<S>(v);
         __syncthreads();
-        const int base = (t >> 4) * 272 + k;
-        float *wr = lre + base, *wi = lim + base;
+        const int base = lds_opaque(lds_re((t >> 4) * 256 + k));
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            wr[17 * r] = v[r].x;
-            wi[17 * r] = v[r].y;
-        }
+        for (int r = 0; r < 16; ++r) lds_put(lds, base ^ lds_re(16 * r), v[r]);
         __syncthreads();
     }
     // pass 3: radix 16, Ns = 256
     {
         const int k = t & 255;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = make_float2(lre[rd + 1088 * r], lim[rd + 1088 * r]);
+        for (int r = 0; r < 16; ++r) v[r] = lds_get(lds, rd + 2048 * r);
         const int m1 = k, m2 = 2 * k, m4 = 4 * k, m8 = 8 * k;   // < 4096
         twiddle16_pow(v, ctw<S>(cmul(tw->h4k[m1 >> 6], tw->l4k[m1 & 63])),
                       ctw<S>(cmul(tw->h4k[m2 >> 6], tw->l4k[m2 & 63])),
@@ -326,19 +362,20 @@ __device__ __forceinline__ void fft16k(float2 *v, float *lre, float *lim, const 
                       ctw<S>(cmul(tw->h4k[m8 >> 6], tw->l4k[m8 & 63])));
         dft16<S>(v);
         __syncthreads();
-        const int base = (t >> 8) * 4352 + k + (k >> 4);
-        float *wr = lre + base, *wi = lim + base;
+        const int base = lds_opaque(lds_re((t >> 8) * 4096 + k));
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            wr[272 * r] = v[r].x;
-            wi[272 * r] = v[r].y;
-        }
+        for (int r = 0; r < 16; ++r) lds_put(lds, (base ^ (lds_re(256 * r) & 63)) + 512 * r, v[r]);
         __syncthreads();
     }
     // pass 4: radix 4, Ns = 4096; butterfly j = t + 1024*b reads/writes j + 4096*r,
     // i.e. element t + 1024*(b + 4r) -> register v[b + 4r]
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
+    for (int q = 0; q < 16; ++q) v[q] = lds_get(lds, rd + 2048 * q);
+}
+
+template <int S>
+__device__ __forceinline__ void fft16k_p4(float2 *v, const FftTw *tw) {
+    const int t = threadIdx.x;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const uint32_t j = (uint32_t)(t + 1024 * b);

@@ -130,13 +130,11 @@ __device__ __forceinline__ int mfma_elem(int g, int j) {
     return 64 * w + 16 * g + (l & 15) + 1024 * (4 * (l >> 4) + j);
 }
 
-__device__ __forceinline__ int lds_pos(int e) { return e + (e >> 4); }
-
 // Radix-16 passes on the matrix cores, then the radix-4 VALU pass.
 // v: the input in MFMA layout (v[4 g + j] = element mfma_elem(g, j));
 // on return v[q] = element threadIdx.x + 1024 q of the transform.
 template <int S, int P>
-__device__ __forceinline__ void fft16k_mfma(float2 *v, float *lre, float *lim, const FftTw *tw,
+__device__ __forceinline__ void fft16k_mfma(float2 *v, float *lds, const FftTw *tw,
                                             const MfmaW<S, P> &W) {
     const int t = threadIdx.x, w = t >> 6, l = t & 63, n = l & 15, kb = l >> 4;
 #pragma unroll
@@ -170,16 +168,14 @@ __device__ __forceinline__ void fft16k_mfma(float2 *v, float *lre, float *lim, c
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) mfma_dft16<S, P>(v + 4 * g, W);
-        __syncthreads();   // previous readers of the LDS arrays are done
+        __syncthreads();   // previous readers of the LDS array are done
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int d = 64 * w + 16 * g + n;
             const int base = (d / ns) * 16 * ns + (d % ns);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int p = lds_pos(base + ns * (4 * kb + i));
-                lre[p] = v[4 * g + i].x;
-                lim[p] = v[4 * g + i].y;
+                lds_put(lds, lds_re(base + ns * (4 * kb + i)), v[4 * g + i]);
             }
         }
         __syncthreads();
@@ -188,15 +184,14 @@ __device__ __forceinline__ void fft16k_mfma(float2 *v, float *lre, float *lim, c
             for (int g = 0; g < 4; ++g)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int p = lds_pos(mfma_elem(g, j));
-                    v[4 * g + j] = make_float2(lre[p], lim[p]);
+                    v[4 * g + j] = lds_get(lds, lds_re(mfma_elem(g, j)));
                 }
         }
     }
     // radix 4, Ns = 4096 (as in fft16k): element t + 1024 (b + 4 r) -> v[b + 4 r]
-    const int rd = t + (t >> 4);
+    const int rd = lds_re(t);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
+    for (int q = 0; q < 16; ++q) v[q] = lds_get(lds, rd + 2048 * q);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const uint32_t j = (uint32_t)(t + 1024 * b);

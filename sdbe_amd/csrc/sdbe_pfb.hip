@@ -55,8 +55,7 @@ static_assert(NH == NCH, "the PFB reuses the 16384-point FFT");
 __global__ void __launch_bounds__(1024)
 k_pfb_fold_r2c(const float *__restrict__ x, const float *__restrict__ win, float2 *__restrict__ spec,
                const float2 *__restrict__ fft_img, const float2 *__restrict__ tw32k) {
-    __shared__ float lre[LDS_PAD_N];
-    __shared__ float lim[LDS_PAD_N];
+    __shared__ float lds[LDS_FFT_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, fft_img);
     const int i = blockIdx.x, t = threadIdx.x;
@@ -76,13 +75,12 @@ k_pfb_fold_r2c(const float *__restrict__ x, const float *__restrict__ win, float
         }
         v[q] = acc;
     }
-    fft16k<-1>(v, lre, lim, &ftw);
+    fft16k<-1>(v, lds, &ftw);
     __syncthreads();    // the FFT's last LDS reads are done
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int k = t + 1024 * q;
-        lre[k + (k >> 4)] = v[q].x;
-        lim[k + (k >> 4)] = v[q].y;
+        lds_put(lds, lds_re(k), v[q]);
     }
     __syncthreads();
     float2 *out = spec + (size_t)i * NBIN;
@@ -90,7 +88,7 @@ k_pfb_fold_r2c(const float *__restrict__ x, const float *__restrict__ win, float
     for (int q = 0; q < 16; ++q) {
         const int k = t + 1024 * q;
         const int km = (NH - k) & (NH - 1);
-        const float2 zm = make_float2(lre[km + (km >> 4)], -lim[km + (km >> 4)]);   // conj Z[N - k]
+        const float2 zm = cconj(lds_get(lds, lds_re(km)));   // conj Z[N - k]
         const float2 E = cscale(cadd(v[q], zm), 0.5f);
         const float2 D = cscale(csub(v[q], zm), 0.5f);
         out[k] = cadd(E, mul_si<-1>(cmul(cconj(tw32k[k]), D)));
@@ -104,8 +102,7 @@ k_pfb_fold_r2c(const float *__restrict__ x, const float *__restrict__ win, float
 __global__ void __launch_bounds__(1024)
 k_pfb_c2r(const float2 *__restrict__ spec, float *__restrict__ yh, const float2 *__restrict__ fft_img,
           const float2 *__restrict__ tw32k) {
-    __shared__ float lre[LDS_PAD_N];
-    __shared__ float lim[LDS_PAD_N];
+    __shared__ float lds[LDS_FFT_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, fft_img);
     const int i = blockIdx.x, t = threadIdx.x;
@@ -122,7 +119,7 @@ k_pfb_c2r(const float2 *__restrict__ spec, float *__restrict__ yh, const float2 
         const float2 sum = cadd(a, cconj(b)), dif = csub(a, cconj(b));
         v[q] = cadd(sum, mul_si<+1>(cmul(tw32k[k], dif)));
     }
-    fft16k<+1>(v, lre, lim, &ftw);
+    fft16k<+1>(v, lds, &ftw);
     float2 *dst = reinterpret_cast<float2 *>(yh + (size_t)i * LB);
     const float sc = 1.0f / (float)LB;
 #pragma unroll

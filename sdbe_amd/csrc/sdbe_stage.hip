@@ -237,8 +237,7 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables
 template <int P>
 __global__ void __launch_bounds__(1024)
 k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int xcd_map) {
-    __shared__ float lre[LDS_PAD_N];
-    __shared__ float lim[LDS_PAD_N];
+    __shared__ float lds[LDS_FFT_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, tb.fft);
     const int lin = blockIdx.x, t = threadIdx.x;
@@ -251,12 +250,15 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
         bpl = lin / NBLK;
         k1 = lin % NBLK;
     }
+    // workgroup-uniform: the row twiddles below are scalar loads
+    k1 = __builtin_amdgcn_readfirstlane(k1);
+    bpl = __builtin_amdgcn_readfirstlane(bpl);
     const float2 *src = V + ((size_t)bpl * NBLK + k1) * NCH;
     float2 v[16];
     if constexpr (P == 0) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = src[t + 1024 * q];
-        fft16k<+1>(v, lre, lim, &ftw);
+        fft16k<+1>(v, lds, &ftw);
     }
     // the MFMA path runs both transforms with the inverse-sign operands: the
     // forward FFT is conj(IFFT(conj(x)))
@@ -267,23 +269,22 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
         for (int g = 0; g < 4; ++g)
 #pragma unroll
             for (int j = 0; j < 4; ++j) v[4 * g + j] = src[mfma_elem(g, j)];
-        fft16k_mfma<+1, P>(v, lre, lim, &ftw, W);
+        fft16k_mfma<+1, P>(v, lds, &ftw, W);
     }
     if constexpr (P == 0) {
         // e^{-2 pi i (t + 1024 q) k1 / M'} = A(t) * B(q); B is uniform over the workgroup
         const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<-1>(tb.nmp, (uint32_t)(1024 * q * k1))));
-        fft16k<-1>(v, lre, lim, &ftw);
+        fft16k<-1>(v, lds, &ftw);
     } else {
         // natural order -> MFMA layout through LDS; the twiddle of element
         // e = d + 1024 r is applied there: e^{-2 pi i e k1 / M'} = A(d) * B(r)
-        const int rd = t + (t >> 4), kb = (t & 63) >> 4;
+        const int rd = lds_re(t), kb = (t & 63) >> 4;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            lre[rd + 1088 * q] = v[q].x;
-            lim[rd + 1088 * q] = v[q].y;
+            lds_put(lds, rd + 2048 * q, v[q]);
         }
         __syncthreads();
         float2 Br[4];
@@ -294,11 +295,10 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0) & 1023) * k1));
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int pp = lds_pos(mfma_elem(g, j));
-                v[4 * g + j] = cconj(cmul(make_float2(lre[pp], lim[pp]), cmul(Ad, Br[j])));
+                v[4 * g + j] = cconj(cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j])));
             }
         }
-        fft16k_mfma<+1, P>(v, lre, lim, &ftw, W);
+        fft16k_mfma<+1, P>(v, lds, &ftw, W);
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = cconj(v[q]);
     }
@@ -601,8 +601,7 @@ template <bool F32, int P>
 __global__ void __launch_bounds__(1024)
 k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const BpDesc *__restrict__ bpd,
            uint8_t *__restrict__ stage, const int *__restrict__ slot_col, Tables tb) {
-    __shared__ float lre[LDS_PAD_N];
-    __shared__ float lim[LDS_PAD_N];
+    __shared__ float lds[LDS_FFT_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, tb.fft);
     const int tx = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
@@ -616,7 +615,7 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
     const float *src = Bc + ((size_t)bpl * KHI + tx) * 2 * BC_ROW;
     constexpr int NSLOT = BC_ROW / 1024;          // 17 full passes ...
     constexpr int REM = BC_ROW - NSLOT * 1024;    // ... and 64 slots left
-    const int rd = t + (t >> 4);
+    const int rd = lds_re(t);
     {
         float pr[NSLOT + 1], pi[NSLOT + 1];
 #pragma unroll
@@ -633,8 +632,7 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             bool valid, dup;
             c1_column(slot >> 5, slot & 31, c, valid, dup);
             if ((i < NSLOT || t < REM) && valid && !dup) {
-                lre[c + (c >> 4)] = pr[i];
-                lim[c + (c >> 4)] = pi[i];
+                lds_put(lds, lds_re(c), make_float2(pr[i], pi[i]));
             }
         }
     }
@@ -642,13 +640,13 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
     float2 v[16];
     if constexpr (P == 0) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = make_float2(lre[rd + 1088 * q], lim[rd + 1088 * q]);
+        for (int q = 0; q < 16; ++q) v[q] = lds_get(lds, rd + 2048 * q);
         // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
         // times a workgroup-uniform one
         const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
-        fft16k<+1>(v, lre, lim, &ftw);
+        fft16k<+1>(v, lds, &ftw);
     } else {
         // MFMA layout: element c = d + 1024 r, twiddle e^{2 pi i c tx/2^20} = A(d) B(r)
         const int l = t & 63, kb = l >> 4;
@@ -661,13 +659,12 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             const float2 Ad = tw<+1>(tb.np, (uint32_t)((e0 & 1023) * tx));
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int pp = lds_pos(mfma_elem(g, j));
-                v[4 * g + j] = cmul(make_float2(lre[pp], lim[pp]), cmul(Ad, Br[j]));
+                v[4 * g + j] = cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j]));
             }
         }
         MfmaW<+1, P> W;
         W.init();
-        fft16k_mfma<+1, P>(v, lre, lim, &ftw, W);
+        fft16k_mfma<+1, P>(v, lds, &ftw, W);
     }
     if (F32) {
         GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * t;

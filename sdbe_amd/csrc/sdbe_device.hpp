@@ -63,8 +63,15 @@ struct Tw2 {
     const float2 *lo;
     int lo_bits;
 };
+// The tables are read through the constant address space: no kernel writes
+// them, and so workgroup-uniform lookups stay scalar loads even after the
+// kernel's own global stores (a persistent kernel's next row).
+typedef float F2v __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(4))) F2v ConstF2;
 template <int S> __device__ __forceinline__ float2 tw(const Tw2 &t, uint32_t idx) {
-    float2 w = cmul(t.hi[idx >> t.lo_bits], t.lo[idx & ((1u << t.lo_bits) - 1u)]);
+    const ConstF2 *hi = (const ConstF2 *)t.hi, *lo = (const ConstF2 *)t.lo;
+    const F2v h = hi[idx >> t.lo_bits], l = lo[idx & ((1u << t.lo_bits) - 1u)];
+    float2 w = cmul(make_float2(h.x, h.y), make_float2(l.x, l.y));
     return S > 0 ? w : cconj(w);
 }
 
@@ -312,20 +319,29 @@ __device__ __forceinline__ int lds_opaque(int x) {
 // LDS into v: the LDS array is free after the next barrier) + fft16k_p4
 // (the radix-4 pass, registers only).  Callers may issue work that needs no
 // LDS between the two.
+// t is threadIdx.x; a persistent caller passes an opaque copy (row_tid) so
+// that the compiler recomputes the per-thread addresses in every row instead
+// of hoisting them out of the row loop and spilling them.
 template <int S>
-__device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *tw);
+__device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *tw, int t);
 template <int S>
-__device__ __forceinline__ void fft16k_p4(float2 *v, const FftTw *tw);
+__device__ __forceinline__ void fft16k_p4(float2 *v, const FftTw *tw, int t);
 
 template <int S>
 __device__ __forceinline__ void fft16k(float2 *v, float *lds, const FftTw *tw) {
-    fft16k_p123<S>(v, lds, tw);
-    fft16k_p4<S>(v, tw);
+    fft16k_p123<S>(v, lds, tw, threadIdx.x);
+    fft16k_p4<S>(v, tw, threadIdx.x);
+}
+
+// threadIdx.x behind a volatile asm: not loop-invariant to the compiler
+__device__ __forceinline__ int row_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
 }
 
 template <int S>
-__device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *tw) {
-    const int t = threadIdx.x;
+__device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *tw, int t) {
     const int rd = lds_re(t);
     // pass 1: radix 16, Ns = 1 (no twiddle)
     dft16<S>(v);
@@ -374,8 +390,7 @@ __device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *
 }
 
 template <int S>
-__device__ __forceinline__ void fft16k_p4(float2 *v, const FftTw *tw) {
-    const int t = threadIdx.x;
+__device__ __forceinline__ void fft16k_p4(float2 *v, const FftTw *tw, int t) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const uint32_t j = (uint32_t)(t + 1024 * b);

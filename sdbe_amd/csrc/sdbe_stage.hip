@@ -592,98 +592,138 @@ struct OutDesc {
     uint8_t *lead;           // [2][8192]
 };
 
-// Stage C2: workgroup = (tx, block-pol).
+// Stage C2: row (tx, block-pol), row index KHI bpl + tx.
 // z[tx + 64 ty] = sum_c Bc[tx][c] e^{2 pi i c tx/2^20} e^{2 pi i c ty/16384} is the packed output:
 // samples (2t, 2t+1) = (Re, Im) z[t], scaled by 2^-36 to numpy units.  Codes
 // per vdif_inout_gpu_thread.cu:577 (v = x - off; ((|v| >= th) | 2) ^ 3 (v < 0));
 // the 4-bit pair code (re | im << 2) of t goes to stage[bp][tx][ty].
+// The workgroups are persistent (one per CU: the FFT's LDS fills a CU) and take
+// rows blockIdx.x, blockIdx.x + gridDim.x, ...  A row arrives in stage C1's
+// tile-major slot order (planar: real plane then imaginary plane) and is
+// scattered into LDS in column order (slot_col: column of each slot, -1 for
+// none); the LDS word of each of a thread's slots is computed once.  The next
+// row's loads are issued when the transform no longer needs LDS (after pass 3)
+// and land while pass 4, the scaling and the stores of the current row run
+// (unconditionally: the last row loads itself again, so the registers are
+// always overwritten and are not live across the transform).
+constexpr int C2_NSLOT = BC_ROW / 1024;          // 17 full slot passes ...
+constexpr int C2_REM = BC_ROW - C2_NSLOT * 1024; // ... and 64 slots left
+constexpr int C2_SLOT_TAB = (C2_NSLOT + 1) * 1024; // slot table entries (padded with -1)
+
+// Loads of one row through a raw buffer resource (base and size in SGPRs, one
+// VGPR offset 4 t, the slot pass in the scalar offset): no per-load address
+// arithmetic on the VALU.  Offsets past the row's 2 BC_ROW floats read 0 (the
+// range check of the resource), which covers the last partial slot pass.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;       // raw 32-bit buffer (gfx9 family)
+
+__device__ __forceinline__ void c2_load(const float *__restrict__ Bc, int row, float (&pr)[C2_NSLOT + 1],
+                                        float (&pi)[C2_NSLOT + 1]) {
+    const int t = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(Bc + (size_t)row * 2 * BC_ROW), 0, 2 * BC_ROW * (int)sizeof(float), BUF_RSRC_WORD3);
+#pragma unroll
+    for (int i = 0; i <= C2_NSLOT; ++i) {
+        pr[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, 4 * t, 4096 * i, 0));
+        pi[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, 4 * t, 4 * BC_ROW + 4096 * i, 0));
+    }
+}
+
 template <bool F32, int P>
 __global__ void __launch_bounds__(1024)
 k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const BpDesc *__restrict__ bpd,
-           uint8_t *__restrict__ stage, const int *__restrict__ slot_col, Tables tb) {
-    __shared__ float lds[LDS_FFT_N];
+           uint8_t *__restrict__ stage, const int *__restrict__ slot_col, Tables tb, int n_rows) {
+    // the work array, then a junk row (words LDS_FFT_N + lane, + 64) that takes
+    // the slots without a column, so the scatter has no branches
+    __shared__ float lds[LDS_FFT_N + 128];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, tb.fft);
-    const int tx = blockIdx.x, bpl = blockIdx.y, t = threadIdx.x;
+    const int t = threadIdx.x;
     const float scale = 1.4551915228366852e-11f;   // 2^-36, exact
-    // The row arrives in stage C1's tile-major slot order (planar: real plane
-    // then imaginary plane) and is scattered into LDS in column order: slot
-    // 32 tile + col holds column c1_column(tile, col) when that is valid and not
-    // a duplicate.  All loads are issued before any LDS store; slots without a
-    // column are read too (the row is allocated) and dropped.
-    (void)slot_col;
-    const float *src = Bc + ((size_t)bpl * KHI + tx) * 2 * BC_ROW;
-    constexpr int NSLOT = BC_ROW / 1024;          // 17 full passes ...
-    constexpr int REM = BC_ROW - NSLOT * 1024;    // ... and 64 slots left
-    const int rd = lds_re(t);
-    {
-        float pr[NSLOT + 1], pi[NSLOT + 1];
+    // LDS word of each slot of this thread, two 16-bit words per register
+    // (slot_col has C2_SLOT_TAB entries, -1 past the row: all loads are issued
+    // before any is used)
+    int cs[C2_NSLOT + 1];
 #pragma unroll
-        for (int i = 0; i < NSLOT; ++i) {
-            pr[i] = src[t + 1024 * i];
-            pi[i] = src[BC_ROW + t + 1024 * i];
+    for (int i = 0; i <= C2_NSLOT; ++i) cs[i] = slot_col[t + 1024 * i];
+    uint32_t pos2[(C2_NSLOT + 2) / 2];
+#pragma unroll
+    for (int i = 0; i <= C2_NSLOT; ++i) {
+        const uint32_t p = (uint32_t)(cs[i] >= 0 ? lds_re(cs[i]) : LDS_FFT_N + (t & 63));
+        if (i & 1)
+            pos2[i >> 1] |= p << 16;
+        else
+            pos2[i >> 1] = p;
+    }
+    float pr[C2_NSLOT + 1], pi[C2_NSLOT + 1];
+    int row = blockIdx.x;
+    c2_load(Bc, row, pr, pi);
+    for (;;) {
+        const int tx = row & (KHI - 1), bpl = row / KHI;
+        __syncthreads();   // the previous row's LDS reads are done
+#pragma unroll
+        for (int i = 0; i <= C2_NSLOT; ++i)
+            lds_put(lds, (i & 1) ? (int)(pos2[i >> 1] >> 16) : (int)(pos2[i >> 1] & 0xffffu),
+                    make_float2(pr[i], pi[i]));
+        __syncthreads();
+        const int next = row + gridDim.x;
+        float2 v[16];
+        if constexpr (P == 0) {
+            const int tr = row_tid();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = lds_get(lds, lds_re(tr) + 2048 * q);
+            // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
+            // times a workgroup-uniform one
+            const float2 A = tw<+1>(tb.np, (uint32_t)(tr * tx));
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
+            fft16k_p123<+1>(v, lds, &ftw, tr);
+            c2_load(Bc, next < n_rows ? next : row, pr, pi);
+            // keep the loads here: the scheduler would otherwise sink them below
+            // the stores at the end of the row, exposing their whole latency
+            __builtin_amdgcn_sched_barrier(0);
+            fft16k_p4<+1>(v, &ftw, tr);
+        } else {
+            // MFMA layout: element c = d + 1024 r, twiddle e^{2 pi i c tx/2^20} = A(d) B(r)
+            const int l = t & 63, kb = l >> 4;
+            float2 Br[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Br[j] = tw<+1>(tb.np, (uint32_t)(1024 * (4 * kb + j) * tx));
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int e0 = mfma_elem(g, 0);
+                const float2 Ad = tw<+1>(tb.np, (uint32_t)((e0 & 1023) * tx));
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[4 * g + j] = cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j]));
+                }
+            }
+            MfmaW<+1, P> W;
+            W.init();
+            fft16k_mfma<+1, P>(v, lds, &ftw, W);
+            c2_load(Bc, next < n_rows ? next : row, pr, pi);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        pr[NSLOT] = t < REM ? src[t + 1024 * NSLOT] : 0.f;
-        pi[NSLOT] = t < REM ? src[BC_ROW + t + 1024 * NSLOT] : 0.f;
+        if (F32) {
+            GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * t;
 #pragma unroll
-        for (int i = 0; i <= NSLOT; ++i) {
-            const int slot = t + 1024 * i;
-            int c;
-            bool valid, dup;
-            c1_column(slot >> 5, slot & 31, c, valid, dup);
-            if ((i < NSLOT || t < REM) && valid && !dup) {
-                lds_put(lds, lds_re(c), make_float2(pr[i], pi[i]));
+            for (int q = 0; q < 16; ++q) {
+                const float2 y = cscale(v[q], scale);
+                dst[(size_t)KHI * 1024 * q] = (uint64_t)__float_as_uint(y.x) | ((uint64_t)__float_as_uint(y.y) << 32);
+            }
+        } else {
+            const float2 to = thr[bpl];
+            uint8_t *dst = stage + (size_t)row * NCH + t;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const float a = v[q].x * scale - to.y;
+                const float b = v[q].y * scale - to.y;
+                const uint32_t ca = ((((fabsf(a) >= to.x) ? 1u : 0u) | 2u) ^ (a < 0.f ? 3u : 0u)) & 3u;
+                const uint32_t cb = ((((fabsf(b) >= to.x) ? 1u : 0u) | 2u) ^ (b < 0.f ? 3u : 0u)) & 3u;
+                dst[1024 * q] = (uint8_t)(ca | (cb << 2));
             }
         }
-    }
-    __syncthreads();
-    float2 v[16];
-    if constexpr (P == 0) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = lds_get(lds, rd + 2048 * q);
-        // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
-        // times a workgroup-uniform one
-        const float2 A = tw<+1>(tb.np, (uint32_t)(t * tx));
-#pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
-        fft16k<+1>(v, lds, &ftw);
-    } else {
-        // MFMA layout: element c = d + 1024 r, twiddle e^{2 pi i c tx/2^20} = A(d) B(r)
-        const int l = t & 63, kb = l >> 4;
-        float2 Br[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) Br[j] = tw<+1>(tb.np, (uint32_t)(1024 * (4 * kb + j) * tx));
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int e0 = mfma_elem(g, 0);
-            const float2 Ad = tw<+1>(tb.np, (uint32_t)((e0 & 1023) * tx));
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                v[4 * g + j] = cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j]));
-            }
-        }
-        MfmaW<+1, P> W;
-        W.init();
-        fft16k_mfma<+1, P>(v, lds, &ftw, W);
-    }
-    if (F32) {
-        GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * t;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const float2 y = cscale(v[q], scale);
-            dst[(size_t)KHI * 1024 * q] = (uint64_t)__float_as_uint(y.x) | ((uint64_t)__float_as_uint(y.y) << 32);
-        }
-    } else {
-        const float2 to = thr[bpl];
-        uint8_t *dst = stage + ((size_t)bpl * KHI + tx) * NCH + t;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const float a = v[q].x * scale - to.y;
-            const float b = v[q].y * scale - to.y;
-            const uint32_t ca = ((((fabsf(a) >= to.x) ? 1u : 0u) | 2u) ^ (a < 0.f ? 3u : 0u)) & 3u;
-            const uint32_t cb = ((((fabsf(b) >= to.x) ? 1u : 0u) | 2u) ^ (b < 0.f ? 3u : 0u)) & 3u;
-            dst[1024 * q] = (uint8_t)(ca | (cb << 2));
-        }
+        if (next >= n_rows) break;
+        row = next;
     }
 }
 
@@ -840,6 +880,7 @@ struct sdbe_handle {
     hipEvent_t g_done = nullptr;
     int g_used = 0;
     int c1_tpw = 8;
+    int n_cu = 256;                  // compute units of the device
     // re-framed group output (output-thread time alignment)
     int align_pending = 0;           // the next group starts an aligned stream
     int align_active = 0;
@@ -971,7 +1012,7 @@ int build_tables(sdbe_handle_t *h) {
 // Slot -> column table of the stage C1 output (same rule as c1_column); every
 // column 0..16383 must own exactly one slot.
 int build_slot_table(sdbe_handle_t *h) {
-    std::vector<int> tab(BC_ROW, -1), seen(NCH, 0);
+    std::vector<int> tab(C2_SLOT_TAB, -1), seen(NCH, 0);
     for (int tile = 0; tile < C1_TILES; ++tile) {
         const int a = tile / C1_CLASS_TILES, u = tile - a * C1_CLASS_TILES;
         for (int col = 0; col < 32; ++col) {
@@ -985,15 +1026,17 @@ int build_slot_table(sdbe_handle_t *h) {
     }
     for (int c = 0; c < NCH; ++c)
         if (seen[c] != 1) return SDBE_ERR_STATE;
-    HIP_TRY(h, hipMemcpy(h->slot_col, tab.data(), BC_ROW * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(h, hipMemcpy(h->slot_col, tab.data(), C2_SLOT_TAB * sizeof(int), hipMemcpyHostToDevice));
     return SDBE_OK;
 }
 
 // Run stages A..C2 over block-pols [0, n_bp) in chunks.
 template <typename K>
 void launch_c2_kernel(sdbe_handle_t *h, int nb, hipStream_t s, const BpDesc *bpd, K kern) {
-    LAUNCH(h, KID_C2, nb, s, kern, dim3(KHI, nb), dim3(1024), (const float *)h->Bc, (const float2 *)h->thr, bpd,
-           h->stage, (const int *)h->slot_col, h->tb);
+    const int rows = KHI * nb;
+    const int grid = rows < h->n_cu ? rows : h->n_cu;
+    LAUNCH(h, KID_C2, nb, s, kern, dim3(grid), dim3(1024), (const float *)h->Bc, (const float2 *)h->thr, bpd,
+           h->stage, (const int *)h->slot_col, h->tb, rows);
 }
 
 template <bool F32>
@@ -1250,6 +1293,11 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
         delete h;
         return SDBE_ERR_NODEV;
     }
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+            h->n_cu = ncu;
+    }
     const size_t C = (size_t)h->chunk;
 #define ALLOC(ptr, bytes)                              \
     do {                                               \
@@ -1269,7 +1317,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->V, C * NBLK * NCH * sizeof(float2));
     ALLOC(h->Zf, C * MP * sizeof(float2));
     ALLOC(h->Bc, C * KHI * BC_ROW * sizeof(float2));
-    ALLOC(h->slot_col, BC_ROW * sizeof(int));
+    ALLOC(h->slot_col, C2_SLOT_TAB * sizeof(int));
     ALLOC(h->part, C * C1_TILES * 2 * sizeof(double));
     ALLOC(h->thr, C * sizeof(float2));
     ALLOC(h->stage, C * KHI * NCH);

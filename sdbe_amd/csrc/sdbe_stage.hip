@@ -227,21 +227,22 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables
     for (int k1 = 0; k1 < NBLK; ++k1) dst[(size_t)k1 * NCH] = x[k1];
 }
 
+// Word 3 of a raw buffer resource (untyped dword access, gfx9 family): loads
+// through it are range-checked against the resource's size and read 0 past it.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+
 // Stage B: row k1 of block-pol bpl: w = IFFT16384(V[k1]); w[n] *= e^{-2 pi i n k1/638976};
 // Zf[k1 + 39 k2] = FFT16384(w)[k2]   (packed forward DFT of the 1,277,952 samples)
-// Workgroups are dispatched round-robin over the 8 XCDs; with xcd_map the 39
-// rows of a block-pol all run on one XCD (requires the chunk to be a multiple
-// of 8).  The packed spectrum is stored row-major: Zf[k1 + 39 k2] at
-// k1 * 16384 + k2.  P = 0: FP32 VALU FFTs; P = FftMode: radix-16 passes on the
-// matrix cores (fft16k_mfma).
-template <int P>
-__global__ void __launch_bounds__(1024)
-k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int xcd_map) {
-    __shared__ float lds[LDS_FFT_N];
-    __shared__ FftTw ftw;
-    fft_tw_load(&ftw, tb.fft);
-    const int lin = blockIdx.x, t = threadIdx.x;
-    int k1, bpl;
+// The packed spectrum is stored row-major: Zf[k1 + 39 k2] at k1 * 16384 + k2.
+// P = 0: FP32 VALU FFTs; P = FftMode: radix-16 passes on the matrix cores
+// (fft16k_mfma).
+// The workgroups are persistent (one per CU: the FFT's LDS fills a CU) and take
+// row slots lin = blockIdx.x, blockIdx.x + gridDim.x, ...  With xcd_map the
+// 39 rows of a block-pol all run on one XCD (workgroups are dealt round-robin
+// over the 8 XCDs; requires the chunk and the grid to be multiples of 8).  The
+// next row is loaded (buffer loads, no VALU address arithmetic) while the last
+// pass, the conjugation and the stores of the current row run.
+__device__ __forceinline__ void b_row(int lin, int xcd_map, int &bpl, int &k1) {
     if (xcd_map) {
         const int slot = lin >> 3;
         bpl = (lin & 7) + 8 * (slot / NBLK);
@@ -250,62 +251,109 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
         bpl = lin / NBLK;
         k1 = lin % NBLK;
     }
-    // workgroup-uniform: the row twiddles below are scalar loads
-    k1 = __builtin_amdgcn_readfirstlane(k1);
-    bpl = __builtin_amdgcn_readfirstlane(bpl);
-    const float2 *src = V + ((size_t)bpl * NBLK + k1) * NCH;
-    float2 v[16];
-    if constexpr (P == 0) {
+}
+
+__device__ __forceinline__ void b_load(const float2 *__restrict__ V, int bpl, int k1, float2 (&x)[16]) {
+    const int t = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(V + ((size_t)bpl * NBLK + k1) * NCH), 0, NCH * (int)sizeof(float2), BUF_RSRC_WORD3);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = src[t + 1024 * q];
-        fft16k<+1>(v, lds, &ftw);
+    for (int q = 0; q < 16; ++q) {
+        const float2 y = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, 8 * t, 8192 * q, 0));
+        x[q] = y;
     }
-    // the MFMA path runs both transforms with the inverse-sign operands: the
-    // forward FFT is conj(IFFT(conj(x)))
-    MfmaW<+1, P> W;
-    if constexpr (P != 0) {
-        W.init();
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) v[4 * g + j] = src[mfma_elem(g, j)];
-        fft16k_mfma<+1, P>(v, lds, &ftw, W);
+}
+
+template <int P>
+__global__ void __launch_bounds__(1024)
+k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int xcd_map, int n_rows) {
+    __shared__ float lds[LDS_FFT_N];
+    __shared__ FftTw ftw;
+    fft_tw_load(&ftw, tb.fft);
+    int lin = blockIdx.x;
+    float2 nx[16];
+    {
+        int bpl, k1;
+        b_row(lin, xcd_map, bpl, k1);
+        b_load(V, bpl, k1, nx);
     }
-    if constexpr (P == 0) {
-        // e^{-2 pi i (t + 1024 q) k1 / M'} = A(t) * B(q); B is uniform over the workgroup
-        const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
+    for (;;) {
+        int k1, bpl;
+        b_row(lin, xcd_map, bpl, k1);
+        // workgroup-uniform: the row twiddles below are scalar loads
+        k1 = __builtin_amdgcn_readfirstlane(k1);
+        bpl = __builtin_amdgcn_readfirstlane(bpl);
+        const int next = lin + (int)gridDim.x;
+        int nbpl, nk1;
+        b_row(next < n_rows ? next : lin, xcd_map, nbpl, nk1);
+        nbpl = __builtin_amdgcn_readfirstlane(nbpl);
+        nk1 = __builtin_amdgcn_readfirstlane(nk1);
+        const int t = row_tid();
+        float2 v[16];
+        if constexpr (P == 0) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<-1>(tb.nmp, (uint32_t)(1024 * q * k1))));
-        fft16k<-1>(v, lds, &ftw);
-    } else {
-        // natural order -> MFMA layout through LDS; the twiddle of element
-        // e = d + 1024 r is applied there: e^{-2 pi i e k1 / M'} = A(d) * B(r)
-        const int rd = lds_re(t), kb = (t & 63) >> 4;
-        __syncthreads();
+            for (int q = 0; q < 16; ++q) v[q] = nx[q];
+            fft16k_p123<+1>(v, lds, &ftw, t);
+            fft16k_p4<+1>(v, &ftw, t);
+            // e^{-2 pi i (t + 1024 q) k1 / M'} = A(t) * B(q); B is uniform over the workgroup
+            const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            lds_put(lds, rd + 2048 * q, v[q]);
-        }
-        __syncthreads();
-        float2 Br[4];
+            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<-1>(tb.nmp, (uint32_t)(1024 * q * k1))));
+            fft16k_p123<-1>(v, lds, &ftw, t);
+            // the next row (unconditionally: the last row loads itself again, so
+            // the registers are always overwritten); keep the loads here, the
+            // scheduler would otherwise sink them below this row's stores
+            b_load(V, nbpl, nk1, nx);
+            __builtin_amdgcn_sched_barrier(0);
+            fft16k_p4<-1>(v, &ftw, t);
+        } else {
+            // the MFMA path runs both transforms with the inverse-sign operands: the
+            // forward FFT is conj(IFFT(conj(x)))
+            MfmaW<+1, P> W;
+            W.init();
+            {
+                const int rd = lds_re(t);
+                __syncthreads();   // the previous row's LDS reads are done
 #pragma unroll
-        for (int j = 0; j < 4; ++j) Br[j] = tw<-1>(tb.nmp, (uint32_t)(1024 * (4 * kb + j) * k1));
+                for (int q = 0; q < 16; ++q) lds_put(lds, rd + 2048 * q, nx[q]);
+                __syncthreads();
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0) & 1023) * k1));
+                for (int g = 0; g < 4; ++g)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                v[4 * g + j] = cconj(cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j])));
+                    for (int j = 0; j < 4; ++j) v[4 * g + j] = lds_get(lds, lds_re(mfma_elem(g, j)));
             }
+            fft16k_mfma<+1, P>(v, lds, &ftw, W);
+            // natural order -> MFMA layout through LDS; the twiddle of element
+            // e = d + 1024 r is applied there: e^{-2 pi i e k1 / M'} = A(d) * B(r)
+            const int rd = lds_re(t), kb = (t & 63) >> 4;
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) lds_put(lds, rd + 2048 * q, v[q]);
+            __syncthreads();
+            float2 Br[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Br[j] = tw<-1>(tb.nmp, (uint32_t)(1024 * (4 * kb + j) * k1));
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0) & 1023) * k1));
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[4 * g + j] = cconj(cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j])));
+                }
+            }
+            fft16k_mfma<+1, P>(v, lds, &ftw, W);
+            b_load(V, nbpl, nk1, nx);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = cconj(v[q]);
         }
-        fft16k_mfma<+1, P>(v, lds, &ftw, W);
+        // row-major [k1][k2]: Zf[k1 + 39 k2] lives at k1 * 16384 + k2
+        float2 *dst = Zf + ((size_t)bpl * NBLK + k1) * NCH + t;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = cconj(v[q]);
+        for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
+        if (next >= n_rows) break;
+        lin = next;
     }
-    // row-major [k1][k2]: Zf[k1 + 39 k2] lives at k1 * 16384 + k2
-    float2 *dst = Zf + ((size_t)bpl * NBLK + k1) * NCH + t;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
 }
 
 // Stage C1: workgroup = 16 column pairs (c, 16384 - c) of the [64][16384] view
@@ -614,7 +662,6 @@ constexpr int C2_SLOT_TAB = (C2_NSLOT + 1) * 1024; // slot table entries (padded
 // VGPR offset 4 t, the slot pass in the scalar offset): no per-load address
 // arithmetic on the VALU.  Offsets past the row's 2 BC_ROW floats read 0 (the
 // range check of the resource), which covers the last partial slot pass.
-constexpr int BUF_RSRC_WORD3 = 0x00020000;       // raw 32-bit buffer (gfx9 family)
 
 __device__ __forceinline__ void c2_load(const float *__restrict__ Bc, int row, float (&pr)[C2_NSLOT + 1],
                                         float (&pi)[C2_NSLOT + 1]) {
@@ -1058,23 +1105,25 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
         const BpDesc *bpd = d_bpd + bp0;
         LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb);
         {
-            const int xm = (nb % 8) == 0 ? 1 : 0;
+            const int rows = NBLK * nb;
+            const int grid = rows < h->n_cu ? rows : h->n_cu;
+            const int xm = (nb % 8) == 0 && (grid % 8) == 0 ? 1 : 0;
             switch (h->cfg.fft_precision) {
             case FFT_BF16X3:
-                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_BF16X3>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V,
-                       h->Zf, h->tb, xm);
+                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_BF16X3>, dim3(grid), dim3(1024), (const float2 *)h->V, h->Zf,
+                       h->tb, xm, rows);
                 break;
             case FFT_BF16:
-                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_BF16>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf,
-                       h->tb, xm);
+                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_BF16>, dim3(grid), dim3(1024), (const float2 *)h->V, h->Zf,
+                       h->tb, xm, rows);
                 break;
             case FFT_FP16:
-                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_FP16>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf,
-                       h->tb, xm);
+                LAUNCH(h, KID_B, nb, s, k_stage_b<FFT_FP16>, dim3(grid), dim3(1024), (const float2 *)h->V, h->Zf,
+                       h->tb, xm, rows);
                 break;
             default:
-                LAUNCH(h, KID_B, nb, s, k_stage_b<0>, dim3(NBLK * nb), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
-                       xm);
+                LAUNCH(h, KID_B, nb, s, k_stage_b<0>, dim3(grid), dim3(1024), (const float2 *)h->V, h->Zf, h->tb,
+                       xm, rows);
             }
         }
         const int c1_wgs = (C1_TILES * nb + h->c1_tpw - 1) / h->c1_tpw;

@@ -396,26 +396,27 @@ __device__ __forceinline__ void c1_fetch(const float2 *__restrict__ Zf, const Ta
     c1_column(x.tile, col, x.c, x.valid, x.dup);
     const int a = x.tile / C1_CLASS_TILES, u = x.tile - a * C1_CLASS_TILES, j16 = col & 15;
     const bool mirror = col >= 16 && x.c != 0;
-    const float2 *zf = Zf + (size_t)x.bpl * MP;
     // m = a + 16384 r + 39 (16 u + j) sits at row (a + 16384 r) mod 39, column
     // (a + 16384 r) / 39 + 16 u + j of the row-major store; the mirror column
-    // 16384 - c at m = 16384 (r + 1) - a - 39 (16 u + j).  Row 39 holds only
-    // m = M' (c = 0), which the store keeps at address 0 (as m = 0); values
-    // that are not used also read address 0, so the loads are unconditional and
-    // all five are in flight together.
+    // 16384 - c at m = 16384 (r + 1) - a - 39 (16 u + j).  With 16384 = 39 * 420 + 4:
+    //   a + 16384 r          = 39 (420 r) + S,         S = a + 4 r
+    //   16384 (r + 1) - a    = 39 (420 r + 419) + S,   S = 4 r + 43 - a
+    // and 0 <= S <= 199, so S / 39 = (S * 1681) >> 16 exactly: no branches and
+    // no divisions.  Row 39 holds only m = M' (c = 0), which the store keeps at
+    // address 0 (as m = 0); values that are not used also read address 0, so the
+    // loads are unconditional and all five are in flight together.
+    const int s0 = (mirror ? 43 - a : a) + 4 * rw;
+    const int c0 = (mirror ? 419 - 16 * u - j16 : 16 * u + j16) + 420 * rw;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(Zf + (size_t)x.bpl * MP), 0, MP * (int)sizeof(float2), BUF_RSRC_WORD3);
 #pragma unroll
     for (int it = 0; it < 5; ++it) {
         const int r = it * 8 + rw;
         const bool ok = x.valid && r < NBLK;
-        int addr;
-        if (mirror) {
-            const int hi = NCH * (r + 1) - a;
-            addr = (hi % NBLK) * NCH + hi / NBLK - 16 * u - j16;
-        } else {
-            const int lo = a + NCH * r;
-            addr = (lo % NBLK) * NCH + lo / NBLK + 16 * u + j16;
-        }
-        z[it] = zf[ok ? addr : 0];
+        const int S = s0 + 32 * it;
+        const int Q = (S * 1681) >> 16;
+        const int addr = (S - NBLK * Q) * NCH + Q + c0 + 3360 * it;   // 3360 = 420 * 8
+        z[it] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, ok ? 8 * addr : 0, 0, 0));
     }
     cm = tb.c1col[x.c];
     cl = tb.c1col[NCH + x.c];
@@ -518,15 +519,21 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
         }
         __syncthreads();
         // write the tile's 32 slots of every row tx: Bc rows are planar (real
-        // plane then imaginary plane, BC_ROW floats each): two 128-byte runs
-        float *bc = Bc + (size_t)x.bpl * KHI * 2 * BC_ROW + (size_t)x.tile * 32;
-#pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            const int tx = it * 8 + rw;
+        // plane then imaginary plane, BC_ROW floats each): two 128-byte runs.
+        // Buffer stores: one VGPR offset, row tx = 8 it + rw in the scalar offset.
+        {
+            const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(Bc + (size_t)x.bpl * KHI * 2 * BC_ROW), 0, KHI * 2 * BC_ROW * (int)sizeof(float),
+                BUF_RSRC_WORD3);
+            const int vo = 4 * (rw * 2 * BC_ROW + x.tile * 32 + col);
             if (x.valid && !x.dup) {
-                const float2 y = sc[col][tx];
-                bc[(size_t)tx * 2 * BC_ROW + col] = y.x;
-                bc[(size_t)tx * 2 * BC_ROW + BC_ROW + col] = y.y;
+#pragma unroll
+                for (int it = 0; it < 8; ++it) {
+                    const float2 y = sc[col][it * 8 + rw];
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(y.x), rb, vo, 4 * (it * 8 * 2 * BC_ROW), 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(y.y), rb, vo,
+                                                          4 * (it * 8 * 2 * BC_ROW + BC_ROW), 0);
+                }
             }
         }
         // Parseval partial: wave reduce then cross-wave
@@ -660,8 +667,9 @@ constexpr int C2_SLOT_TAB = (C2_NSLOT + 1) * 1024; // slot table entries (padded
 
 // Loads of one row through a raw buffer resource (base and size in SGPRs, one
 // VGPR offset 4 t, the slot pass in the scalar offset): no per-load address
-// arithmetic on the VALU.  Offsets past the row's 2 BC_ROW floats read 0 (the
-// range check of the resource), which covers the last partial slot pass.
+// arithmetic on the VALU.  Every access is inside the row by construction: in
+// the last, partial slot pass the lanes past the row read the pass's first
+// slot (their values go to the junk row).
 
 __device__ __forceinline__ void c2_load(const float *__restrict__ Bc, int row, float (&pr)[C2_NSLOT + 1],
                                         float (&pi)[C2_NSLOT + 1]) {
@@ -669,10 +677,13 @@ __device__ __forceinline__ void c2_load(const float *__restrict__ Bc, int row, f
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(Bc + (size_t)row * 2 * BC_ROW), 0, 2 * BC_ROW * (int)sizeof(float), BUF_RSRC_WORD3);
 #pragma unroll
-    for (int i = 0; i <= C2_NSLOT; ++i) {
+    for (int i = 0; i < C2_NSLOT; ++i) {
         pr[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, 4 * t, 4096 * i, 0));
         pi[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, 4 * t, 4 * BC_ROW + 4096 * i, 0));
     }
+    const int tl = t < C2_REM ? 4 * t : 0;
+    pr[C2_NSLOT] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, tl, 4096 * C2_NSLOT, 0));
+    pi[C2_NSLOT] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, tl, 4 * BC_ROW + 4096 * C2_NSLOT, 0));
 }
 
 template <bool F32, int P>

@@ -118,7 +118,22 @@ def main():
                     help="FFT arithmetic: 0 FP32 (default), 1 BF16X3, 2 BF16, 3 FP16 matrix-core passes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 3 device copies of 256 MiB (the calibration workload of the "
+                         "FETCH_SIZE / WRITE_SIZE byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 256 << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copied 3 x %d bytes" % n)
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: every GPU box here has one GPU, and blocks are "
               "independent (N GPUs = N disjoint block ranges, no exchange); run with --gpus 1",

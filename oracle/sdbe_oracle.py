@@ -886,3 +886,32 @@ class OutputAligner:
                 self.secs += 1
             self.psn += 1
         return frames, headers
+
+
+# ---------------------------------------------------------------------------
+# 2015-format B-engine reorder
+# ---------------------------------------------------------------------------
+REORDER2015_SPLIT = 69
+
+
+def reorder_2015(x):
+    """2015-format reorder of complex spectra x[channel][B-frame][snapshot]
+    (16384 x n_frames x 128) to out[frame][snapshot][16385], the numpy form of
+    software/prototyping/cuda/reorder.py:284-293: channels 0-3 of every 8 are
+    rolled back two snapshots within their B-frame, then snapshots 69..127 of
+    every channel are taken from the next B-frame, the last frame is dropped,
+    and the result is transposed (reorder.py:291-292 compares it with
+    reorderTz_smem, kernel_template.py:107-164, whose extra channel 16384 is
+    zero).
+    """
+    x = np.asarray(x)
+    nch, nf, nsn = x.shape
+    r = np.empty_like(x)
+    for i in range(4):                                   # reorder.py:285-287
+        r[i::8] = np.roll(x[i::8], -2, axis=-1)
+        r[i + 4::8] = x[i + 4::8]
+    r[:, :, REORDER2015_SPLIT:] = np.roll(r[:, :, REORDER2015_SPLIT:], -1, axis=-2)   # :290
+    r = r[:, :-1, :]                                      # :293
+    out = np.zeros((nf - 1, nsn, nch + 1), dtype=x.dtype)
+    out[:, :, :nch] = r.transpose(1, 2, 0)
+    return out

@@ -118,9 +118,12 @@ class KernelTime(C.Structure):
 PFB_LBLOCK = 32768
 PFB_NBINS = 16385
 PFB_NTAP = 4
+REORDER2015_CHANNELS = 16384
+REORDER2015_SNAPSHOTS = 128
+REORDER2015_SPLIT = 69
 
-# (name, restype, argtypes) of every symbol declared in include/sdbe_stage.h
-# and include/sdbe_pfb.h
+# (name, restype, argtypes) of every symbol declared in include/sdbe_stage.h,
+# include/sdbe_pfb.h and include/sdbe_formats.h
 PROTOTYPES = [
     ("sdbe_default_config", C.c_int, [C.POINTER(Config)]),
     ("sdbe_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
@@ -150,6 +153,8 @@ PROTOTYPES = [
     ("sdbe_pfb_inverse", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("sdbe_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("sdbe_profile_read", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int32, C.POINTER(C.c_int32)]),
+    # include/sdbe_formats.h
+    ("sdbe_reorder_2015", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None

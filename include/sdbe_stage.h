@@ -96,6 +96,24 @@ extern "C" {
 #define SDBE_OUT_PAYLOAD 0
 #define SDBE_OUT_FRAMES  1
 
+/* Resampler.  SDBE_RATE_64_39: the 39-to-64 rational resampler of the
+ * prototypes (resample.py fft_batched): a block of 39 snapshots becomes
+ * 2,097,152 samples, 64 frames per block-pol.  SDBE_RATE_6_11 / _8_11: the
+ * band-trimmed path of the aphids GPU thread compiled for that SWARM rate
+ * (vdif_inout_gpu_thread.cu:589-709, plans :269-333, sample_rates.h:21-40):
+ * per block of D snapshots, a 32768-point C2R per snapshot, 128 real DFTs of
+ * 256 D samples, and from each of them bins skip_chan .. skip_chan + 128 E
+ * into a 256 E-point C2R, skip_chan = skip_mhz 1e6 / (SWARM_RATE / 256 D).
+ *   6/11: D = 39, E = 32: 2496 -> 2048 Msps, 1,048,576 samples, 32 frames per block-pol;
+ *   8/11: D = 13, E = 8:  3328 -> 2048 Msps,   262,144 samples,  8 frames per block-pol.
+ * In a band-trimmed handle a "block-pol" of the calls below is D snapshots of
+ * one polarisation; sdbe_process_blocks and sdbe_set_output_alignment return
+ * SDBE_ERR_STATE.  The 10/11 and 11/11 presets, marked unconfirmed and
+ * over-memory in the reference, are not provided.                          */
+#define SDBE_RATE_64_39 0
+#define SDBE_RATE_6_11  6
+#define SDBE_RATE_8_11  8
+
 typedef struct sdbe_config {
     int32_t thresh_policy;      /* SDBE_THRESH_*                                */
     float   fixed_threshold;    /* used with SDBE_THRESH_FIXED                  */
@@ -113,7 +131,12 @@ typedef struct sdbe_config {
     int32_t max_batch_blockpols;/* capacity of sdbe_process_blocks (0 = 1024)   */
     int32_t fft_precision;      /* SDBE_FFT_*: arithmetic of the 16384-point FFT
                                    passes of stages B and C2                   */
-    int32_t reserved[4];
+    int32_t rate_frac;          /* resampler: SDBE_RATE_64_39 (0), or the aphids
+                                   band-trimmed path of a SWARM rate preset,
+                                   SDBE_RATE_6_11 / SDBE_RATE_8_11             */
+    int32_t skip_mhz;           /* band-trimmed path: MHz skipped at the bottom
+                                   of the band (aphids_resampler_init skip_mhz) */
+    int32_t reserved[2];
 } sdbe_config_t;
 
 /* VDIF output time state for the next frame to be produced; both

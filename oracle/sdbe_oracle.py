@@ -915,3 +915,52 @@ def reorder_2015(x):
     out = np.zeros((nf - 1, nsn, nch + 1), dtype=x.dtype)
     out[:, :, :nch] = r.transpose(1, 2, 0)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Band-trimmed rational resampler of the aphids GPU thread at reduced SWARM
+# rates (software/aphids/sample_rates.h:12-40, vdif_inout_gpu_thread.h:31-53)
+# ---------------------------------------------------------------------------
+SWARM_RATE_FULL = 4576e6                  # sample_rates.h:17
+TRIM_RATES = {6: (39, 32), 8: (13, 8)}    # SWARM_RATE_FRAC -> (DECIMATION, EXPANSION), sample_rates.h:21-40
+TRIM_SCALE = 256                          # FFT_SIZE_SWARM_R2C_R2DBE_C2R_SCALE, vdif_inout_gpu_thread.h:45
+
+
+def trim_geometry(rate_frac, skip_mhz):
+    """(D, E, R, P, skip_chan) of a rate preset: D snapshots per block
+    (RESAMPLE_BATCH_SNAPSHOTS = DECIMATION_FACTOR), R = 256 D the R2C size,
+    P = 256 E the C2R size, skip_chan = int(skip_mhz 1e6 / (SWARM_RATE / R))
+    (vdif_inout_gpu_thread.cu:269, SWARM_RATE = 4576e6 frac / 11)."""
+    D, E = TRIM_RATES[rate_frac]
+    R, P = TRIM_SCALE * D, TRIM_SCALE * E
+    swarm_rate = SWARM_RATE_FULL * rate_frac / 11
+    skip = int(skip_mhz * 1e6 / (swarm_rate / R))
+    if skip < 0 or skip + P // 2 > R // 2:
+        raise ValueError("skip_chan %d leaves fewer than %d bins" % (skip, P // 2 + 1))
+    return D, E, R, P, skip
+
+
+def resample_trimmed(X, E, skip, scale=TRIM_SCALE):
+    """Band-trimmed resampler of one block: X complex spectra [D][nchan]
+    (channel nchan, the SWARM Nyquist, is 0), D snapshots, in numpy units.
+
+    SwarmC2R (vdif_inout_gpu_thread.cu:589-625): an inverse real DFT of
+    2 nchan points per snapshot (numpy irfft semantics: the imaginary parts of
+    bin 0 and of the Nyquist bin are discarded); the D series concatenate.
+    SwarmR2C (:631-667): real DFTs of R = scale D samples, batched over the
+    concatenation (FFT_BATCHES_SWARM_R2C segments).  R2dbeC2R (:673-709, plan
+    :317-333): the inverse real DFT of P = scale E points reads bins
+    skip .. skip + P/2 of each segment's spectrum (input pointer + skip_chan,
+    batch stride R/2 + 1).  Returns the float64 samples, segment after
+    segment (P per segment), normalised as numpy's irfft/rfft/irfft.
+    """
+    X = np.asarray(X)
+    D, nchan = X.shape
+    n1 = 2 * nchan
+    spec = np.concatenate([X, np.zeros((D, 1), dtype=X.dtype)], axis=1)
+    t = np.fft.irfft(spec, n=n1, axis=1).reshape(-1)
+    R, P = scale * D, scale * E
+    nseg = t.size // R
+    Xs = np.fft.rfft(t[:nseg * R].reshape(nseg, R), axis=1)
+    Y = Xs[:, skip:skip + P // 2 + 1]
+    return np.fft.irfft(Y, n=P, axis=1).reshape(-1)

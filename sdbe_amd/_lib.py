@@ -21,6 +21,9 @@ SDBE_ERR_STATE = -6
 THRESH_REFERENCE = 0
 THRESH_PER_BLOCK = 1
 THRESH_FIXED = 2
+RATE_64_39 = 0
+RATE_6_11 = 6
+RATE_8_11 = 8
 FFT_FP32 = 0
 FFT_BF16X3 = 1
 FFT_BF16 = 2
@@ -31,6 +34,7 @@ OUT_FRAMES = 1
 CHANNELS = 16384
 FRAMES_PER_GROUP = 39
 BLOCKS_PER_GROUP = 128
+SNAPSHOTS_PER_GROUP = FRAMES_PER_GROUP * 128     # 4992
 SAMPLES_PER_BLOCK = 2097152
 OUT_FRAMES_PER_BLOCK = 64
 OUT_PAYLOAD_BYTES = 8192
@@ -54,7 +58,9 @@ class Config(C.Structure):
         ("c1_tiles_per_wg", C.c_int32),
         ("max_batch_blockpols", C.c_int32),
         ("fft_precision", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("rate_frac", C.c_int32),
+        ("skip_mhz", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

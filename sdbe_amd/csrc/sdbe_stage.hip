@@ -831,13 +831,13 @@ k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb
 
 // VDIF output headers (vdif_out_databuf.h:19-52) stamped as the output thread
 // does (vdif_out_net_thread.c:294-317, defaults vdif_out_databuf.c:195-227):
-// frame n of a block-pol whose first frame is (secs, df, psn) has
+// frame n (of fpb per block-pol) of a block-pol whose first frame is (secs, df, psn) has
 // df = (df + n) mod 125000, secs + carry, psn + n; threadID 0, bps = 1,
 // df_len = 8224 / 8, edv5 = 2 (words 3 and 4 come prebuilt per block-pol).
-__global__ void __launch_bounds__(256) k_headers(const BpDesc *__restrict__ bpd, int n_bp, OutDesc od) {
+__global__ void __launch_bounds__(256) k_headers(const BpDesc *__restrict__ bpd, int n_bp, OutDesc od, int fpb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_bp * OUT_FRAMES_BLOCK) return;
-    const int b = i / OUT_FRAMES_BLOCK, n = i - b * OUT_FRAMES_BLOCK;
+    if (i >= n_bp * fpb) return;
+    const int b = i / fpb, n = i - b * fpb;
     const BpDesc &d = bpd[b];
     const uint64_t dft = (uint64_t)d.df + (uint64_t)n;
     const uint32_t df = (uint32_t)(dft % 125000u);
@@ -882,16 +882,19 @@ __global__ void __launch_bounds__(256) k_lead_copy(const uint8_t *__restrict__ l
         *reinterpret_cast<const GAS uint32_t *>(gptr(lead) + pol * OUT_FRAME_PAYLOAD + i);
 }
 
+#include "sdbe_trim.hpp"
+
 }  // namespace
 
 // ======================================================================= host
 enum KernelId {
     KID_FILL, KID_MAP, KID_REORDER, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
-    KID_PACK, KID_HEADERS, KID_N
+    KID_PACK, KID_HEADERS, KID_TRIM_C2R, KID_TRIM_SEG, KID_TRIM_PACK, KID_N
 };
 static const char *const kKernelNames[KID_N] = {
     "k_fill_i32", "k_packet_map", "k_reorder", "k_stage_a", "k_stage_b",
-    "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers"};
+    "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers", "k_trim_c2r", "k_trim_seg",
+    "k_trim_pack"};
 
 struct ProfRec {
     int kid;
@@ -939,6 +942,12 @@ struct sdbe_handle {
     int g_used = 0;
     int c1_tpw = 8;
     int n_cu = 256;                  // compute units of the device
+    // resampler: 64/39 (trim.D == 0) or the band-trimmed path of a rate preset
+    trim::Geometry trim;
+    trim::Tabs trim_tabs{};
+    float2 *trim_tab = nullptr;
+    int snaps = NBLK;                // snapshots per block-pol
+    int fpb = OUT_FRAMES_BLOCK;      // output frames per block-pol
     // re-framed group output (output-thread time alignment)
     int align_pending = 0;           // the next group starts an aligned stream
     int align_active = 0;
@@ -1067,6 +1076,31 @@ int build_tables(sdbe_handle_t *h) {
     return SDBE_OK;
 }
 
+// Twiddle tables of the band-trimmed path (trim::Tabs), FP64 on the host.
+int build_trim_tables(sdbe_handle_t *h) {
+    const int NR = h->trim.R / 2, NP = h->trim.P / 2;
+    const size_t n = (size_t)NR + (NR + 1) + NP + NP;
+    std::vector<float2> host(n);
+    auto e = [](double ang) { return make_float2((float)cos(ang), (float)sin(ang)); };
+    const double tau = 6.283185307179586476925286766559;
+    size_t o = 0;
+    const size_t zr = o;
+    for (int m = 0; m < NR; ++m) host[o++] = e(-tau * m / NR);
+    const size_t xr = o;
+    for (int k = 0; k <= NR; ++k) host[o++] = e(-tau * k / h->trim.R);
+    const size_t wp = o;
+    for (int j = 0; j < NP; ++j) host[o++] = e(tau * j / h->trim.P);
+    const size_t zp = o;
+    for (int m = 0; m < NP; ++m) host[o++] = e(tau * m / NP);
+    HIP_TRY(h, hipMalloc(&h->trim_tab, n * sizeof(float2)));
+    HIP_TRY(h, hipMemcpy(h->trim_tab, host.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    h->trim_tabs.zr = h->trim_tab + zr;
+    h->trim_tabs.xr = h->trim_tab + xr;
+    h->trim_tabs.wp = h->trim_tab + wp;
+    h->trim_tabs.zp = h->trim_tab + zp;
+    return SDBE_OK;
+}
+
 // Slot -> column table of the stage C1 output (same rule as c1_column); every
 // column 0..16383 must own exactly one slot.
 int build_slot_table(sdbe_handle_t *h) {
@@ -1107,10 +1141,50 @@ void launch_c2(sdbe_handle_t *h, int nb, hipStream_t s, const BpDesc *bpd) {
     }
 }
 
+// The band-trimmed resampler over n_bp blocks of D snapshots, in chunks: the
+// per-snapshot C2R into V (as the concatenated series), the segment R2C / C2R
+// into Zf (float samples) or the caller's buffers (f32), statistics, the
+// threshold policy and the natural-order packing.
+template <int D, int E>
+int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32, double *d_stats,
+             hipStream_t s) {
+    constexpr int P = 256 * E;
+    const auto kc2r = k_trim_c2r<D>;
+    const auto kseg_f = k_trim_seg<D, E, true>;
+    const auto kseg_c = k_trim_seg<D, E, false>;
+    for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
+        const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
+        const BpDesc *bpd = d_bpd + bp0;
+        LAUNCH(h, KID_TRIM_C2R, nb, s, kc2r, dim3(D * nb), dim3(1024), bpd, h->V, h->tb);
+        if (f32)
+            LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_f, dim3(trim::NSEG * nb), dim3(trim::SEG_THREADS),
+                   (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip);
+        else
+            LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_c, dim3(trim::NSEG * nb), dim3(trim::SEG_THREADS),
+                   (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip);
+        LAUNCH(h, KID_STATS, nb, s, k_trim_stats, dim3(nb), dim3(trim::NSEG), (const double *)h->part, P, bp0, h->ms,
+               d_stats);
+        if (!f32) {
+            LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(256), (const double2 *)h->ms, nb, bpd, h->latch,
+                   h->thr);
+            LAUNCH(h, KID_TRIM_PACK, nb, s, k_trim_pack, dim3(nb * (trim::NSEG * P / 16 / 256)), dim3(256),
+                   (const float *)h->Zf, (const float2 *)h->thr, bpd, P, od);
+        }
+        HIP_TRY(h, hipGetLastError());
+    }
+    if (!f32 && od.hdr_bytes && od.bp_headers) {
+        LAUNCH(h, KID_HEADERS, n_bp, s, k_headers, dim3((n_bp * E + 255) / 256), dim3(256), d_bpd, n_bp, od, E);
+        HIP_TRY(h, hipGetLastError());
+    }
+    return SDBE_OK;
+}
+
 // Run stages A..C2 (+ pack) over the n_bp block-pols described by the device
 // table d_bpd, in chunks.  f32: write float samples instead of codes.
 int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32,
                   double *d_stats, hipStream_t s) {
+    if (h->trim.frac == 6) return run_trim<39, 32>(h, d_bpd, n_bp, od, f32, d_stats, s);
+    if (h->trim.frac == 8) return run_trim<13, 8>(h, d_bpd, n_bp, od, f32, d_stats, s);
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
@@ -1153,7 +1227,7 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
     }
     if (!f32 && od.hdr_bytes && od.bp_headers) {
         LAUNCH(h, KID_HEADERS, n_bp, s, k_headers, dim3((n_bp * OUT_FRAMES_BLOCK + 255) / 256), dim3(256), d_bpd,
-               n_bp, od);
+               n_bp, od, OUT_FRAMES_BLOCK);
         HIP_TRY(h, hipGetLastError());
     }
     return SDBE_OK;
@@ -1298,6 +1372,7 @@ void sdbe_destroy(sdbe_handle_t *h) {
     (void)hipFree(h->thr);
     (void)hipFree(h->stage);
     (void)hipFree(h->slot_col);
+    (void)hipFree(h->trim_tab);
     (void)hipFree(h->ms);
     (void)hipHostFree(h->h_cnt);
     (void)hipHostFree(h->h_latch);
@@ -1332,6 +1407,12 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     if (cfg.max_spectra_blockpols < 1 || cfg.max_batch_blockpols < 0) return SDBE_ERR_ARG;
     if (cfg.c1_tiles_per_wg < 0 || cfg.c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
     if (cfg.fft_precision < SDBE_FFT_FP32 || cfg.fft_precision > SDBE_FFT_FP16) return SDBE_ERR_ARG;
+    trim::Geometry tg;
+    if (cfg.rate_frac != SDBE_RATE_64_39) {
+        // the band-trimmed path runs its own FP32 transforms
+        if (!trim::geometry(cfg.rate_frac, cfg.skip_mhz, tg) || cfg.fft_precision != SDBE_FFT_FP32)
+            return SDBE_ERR_ARG;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SDBE_ERR_NODEV;
     sdbe_handle_t *h = new (std::nothrow) sdbe_handle();
@@ -1340,11 +1421,17 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     h->cfg = cfg;
     h->chunk = cfg.chunk_blockpols;
     h->c1_tpw = cfg.c1_tiles_per_wg ? cfg.c1_tiles_per_wg : C1_TILES_PER_WG;
+    h->trim = tg;
+    if (tg.D) {
+        h->snaps = tg.D;
+        h->fpb = tg.E;
+    }
     h->cap_bp = 2 * SDBE_BLOCKS_PER_GROUP;
     if (cfg.max_spectra_blockpols > h->cap_bp) h->cap_bp = cfg.max_spectra_blockpols;
     {
         const int mb = cfg.max_batch_blockpols ? cfg.max_batch_blockpols : 1024;
         if (mb > h->cap_bp) h->cap_bp = mb;
+        if (tg.D && 2 * (NSNAP_GROUP / tg.D) > h->cap_bp) h->cap_bp = 2 * (NSNAP_GROUP / tg.D);
     }
     h->st.psn = 1;
     int rc = SDBE_OK;
@@ -1407,6 +1494,10 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     if (rc != SDBE_OK) goto fail;
     rc = build_slot_table(h);
     if (rc != SDBE_OK) goto fail;
+    if (h->trim.D) {
+        rc = build_trim_tables(h);
+        if (rc != SDBE_OK) goto fail;
+    }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, h->cnt, CNT_N, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, (int *)h->latch, 2 * SDBE_LATCH_SLOTS, 0);
     if (hipDeviceSynchronize() != hipSuccess) {
@@ -1483,22 +1574,23 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     if (!h || !d_out || (n_pkts > 0 && !d_pkts)) return SDBE_ERR_ARG;
     if (n_pkts < 0 || n_pkts > MAP_N) return SDBE_ERR_CAPACITY;
     OutDesc od = out_desc(h);
-    if (out_pol_stride_bytes < (int64_t)SDBE_OUT_FRAMES_PER_GROUP * od.frame_bytes) return SDBE_ERR_ARG;
+    const int blocks = NSNAP_GROUP / h->snaps;           // per polarisation
+    if (out_pol_stride_bytes < (int64_t)blocks * h->fpb * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
     int rc = unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;
     int slot;
     if ((rc = ring_acquire(h, slot))) return rc;
-    const int n_bp = 2 * SDBE_BLOCKS_PER_GROUP;
+    const int n_bp = 2 * blocks;
     BpDesc *hb = h->h_bpd[slot];
     const sdbe_config_t &c = h->cfg;
     for (int bp = 0; bp < n_bp; ++bp) {
         const int blk = bp >> 1, pol = bp & 1;
-        const int8_t *rows = h->beng + ((size_t)pol * NSNAP_GROUP + (size_t)NBLK * blk) * NCH * 2;
+        const int8_t *rows = h->beng + ((size_t)pol * NSNAP_GROUP + (size_t)h->snaps * blk) * NCH * 2;
         uint8_t *out = (uint8_t *)d_out + (int64_t)pol * out_pol_stride_bytes +
-                       (int64_t)blk * OUT_FRAMES_BLOCK * od.frame_bytes;
-        set_bp(hb[bp], rows, out, state_plus(h->st, (int64_t)blk * OUT_FRAMES_BLOCK), pol, 0, c.thresh_policy, 0,
+                       (int64_t)blk * h->fpb * od.frame_bytes;
+        set_bp(hb[bp], rows, out, state_plus(h->st, (int64_t)blk * h->fpb), pol, 0, c.thresh_policy, 0,
                c.fixed_threshold, c.fixed_offset, c.station_id, c.bdc_sideband, c.rx_sideband);
         hb[bp].blk = blk;
     }
@@ -1507,7 +1599,7 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     // into the first aligned group; a group's last lead_words words open the
     // next group's first frame (the output thread's time alignment,
     // vdif_out_net_thread.c:203-290, done before the copy to the host).
-    int n_out = SDBE_OUT_FRAMES_PER_GROUP;
+    int n_out = blocks * h->fpb;
     if (h->align_pending || h->align_active) {
         int64_t sigma;
         if (h->align_pending) {
@@ -1567,8 +1659,8 @@ int spectra_call(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp, uint8_t *
     BpDesc *hb = h->h_bpd[slot];
     const sdbe_config_t &c = h->cfg;
     for (int bp = 0; bp < n_bp; ++bp)
-        set_bp(hb[bp], d_beng + (size_t)bp * NBLK * NCH * 2, out + (int64_t)bp * out_bp_stride,
-               state_plus(h->st, (int64_t)bp * OUT_FRAMES_BLOCK), 0, 0, c.thresh_policy, 0, c.fixed_threshold,
+        set_bp(hb[bp], d_beng + (size_t)bp * h->snaps * NCH * 2, out + (int64_t)bp * out_bp_stride,
+               state_plus(h->st, (int64_t)bp * h->fpb), 0, 0, c.thresh_policy, 0, c.fixed_threshold,
                c.fixed_offset, c.station_id, c.bdc_sideband, c.rx_sideband);
     if ((rc = ring_upload(h, slot, n_bp, s))) return rc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h->cnt, CNT_N, 0);
@@ -1584,11 +1676,10 @@ int sdbe_process_spectra(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp, v
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
     const OutDesc od = out_desc(h);
-    int rc = spectra_call(h, d_beng, n_bp, (uint8_t *)d_out, (int64_t)OUT_FRAMES_BLOCK * od.frame_bytes, false,
-                          nullptr, s);
+    int rc = spectra_call(h, d_beng, n_bp, (uint8_t *)d_out, (int64_t)h->fpb * od.frame_bytes, false, nullptr, s);
     if (rc) return rc;
-    advance_state(h->st, (int64_t)n_bp * OUT_FRAMES_BLOCK);
-    h->n_frames_last = n_bp * OUT_FRAMES_BLOCK;
+    advance_state(h->st, (int64_t)n_bp * h->fpb);
+    h->n_frames_last = n_bp * h->fpb;
     h->n_packets_last = 0;
     h->n_bp_last = n_bp;
     h->have_group = 0;
@@ -1601,8 +1692,8 @@ int sdbe_resample_spectra_f32(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_
     if (n_bp > h->cfg.max_spectra_blockpols) return SDBE_ERR_CAPACITY;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    int rc = spectra_call(h, d_beng, n_bp, (uint8_t *)d_samples, (int64_t)P20 * 2 * sizeof(float), true, d_stats,
-                          s);
+    const int64_t spb = h->trim.D ? (int64_t)trim::NSEG * h->trim.P : (int64_t)P20 * 2;   // samples per block-pol
+    int rc = spectra_call(h, d_beng, n_bp, (uint8_t *)d_samples, spb * (int64_t)sizeof(float), true, d_stats, s);
     if (rc) return rc;
     h->n_packets_last = 0;
     h->n_bp_last = n_bp;
@@ -1686,6 +1777,7 @@ int batch_graph(sdbe_handle_t *h, int n_bp) {
 }  // namespace
 
 int sdbe_process_blocks(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int32_t n_streams, void *stream) {
+    if (h && h->trim.D) return SDBE_ERR_STATE;   // ragged batches: 64/39 resampler only
     if (!h) return SDBE_ERR_ARG;
     int n_bp = 0;
     int rc = check_batch(h, streams, n_streams, n_bp);
@@ -1719,6 +1811,7 @@ int sdbe_process_blocks(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int
 }
 
 int sdbe_set_output_alignment(sdbe_handle_t *h, int32_t enable, int32_t skip_int32, int32_t skip_frames) {
+    if (h && h->trim.D) return SDBE_ERR_STATE;   // re-framed output: 64/39 resampler only
     if (!h) return SDBE_ERR_ARG;
     if (!enable) {
         h->align_pending = h->align_active = 0;

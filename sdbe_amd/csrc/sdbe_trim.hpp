@@ -1,0 +1,292 @@
+// Band-trimmed rational resampler of the aphids GPU thread at a reduced SWARM
+// rate (software/aphids/vdif_inout_gpu_thread.cu:589-709, plans :269-333;
+// presets sample_rates.h:21-40): per block of D snapshots
+//   SwarmC2R  : inverse real DFT of 32768 points per snapshot (k_trim_c2r),
+//   SwarmR2C  : real DFTs of R = 256 D samples over the concatenation, 128 segments,
+//   R2dbeC2R  : inverse real DFT of P = 256 E points from bins skip .. skip + P/2
+//               of each segment's spectrum (k_trim_seg),
+// then per-block statistics (k_trim_stats), the threshold policy (k_thresh) and
+// 2-bit packing in natural sample order (k_trim_pack).
+//
+// Both real transforms of a segment use the packed form: the R-point R2C is an
+// R/2-point complex FFT of z[m] = t[2m] + i t[2m+1] followed by
+//   X[k] = 1/2 (Z[k] + conj Z[R/2-k]) - 1/2 i e^{-2 pi i k/R} (Z[k] - conj Z[R/2-k]),
+// and the P-point C2R an P/2-point inverse FFT of
+//   W[j] = (Y[j] + conj Y[P/2-j]) + i e^{2 pi i j/P} (Y[j] - conj Y[P/2-j]),
+// Y[j] = X[skip + j] with the imaginary parts of Y[0] and Y[P/2] dropped (numpy
+// irfft semantics, as the rest of the stage).  The R/2- and P/2-point FFTs are
+// mixed-radix Stockham passes through two LDS buffers (4992 = 16 8 13 3,
+// 1664 = 16 8 13, 4096 = 16 16 16, 1024 = 16 8 8).
+//
+// Units: the unnormalised chain is numpy's irfft / rfft / irfft times
+// 32768 P; k_trim_seg removes that exact power of two, so samples, statistics
+// and thresholds are in numpy units.
+#pragma once
+
+namespace trim {
+
+constexpr int NSEG = 128;             // segments per block (D 32768 / (256 D))
+constexpr int SEG_THREADS = 256;
+
+// Host: geometry of a preset.  Returns false for an unsupported preset or a
+// skip that leaves fewer than P/2 + 1 bins.
+struct Geometry {
+    int frac = 0, D = 0, E = 0, R = 0, P = 0, skip = 0;
+};
+inline bool geometry(int rate_frac, int skip_mhz, Geometry &g) {
+    int D, E;
+    if (rate_frac == 6) {
+        D = 39;
+        E = 32;
+    } else if (rate_frac == 8) {
+        D = 13;
+        E = 8;
+    } else {
+        return false;
+    }
+    g.frac = rate_frac;
+    g.D = D;
+    g.E = E;
+    g.R = 256 * D;
+    g.P = 256 * E;
+    const double swarm_rate = 4576e6 * rate_frac / 11.0;
+    g.skip = (int)(skip_mhz * 1e6 / (swarm_rate / g.R));   // vdif_inout_gpu_thread.cu:269
+    return skip_mhz >= 0 && g.skip + g.P / 2 <= g.R / 2;
+}
+
+// DFT of radix R in place, natural order in and out, sign S.
+template <int R, int S>
+__device__ __forceinline__ void dft_r(float2 *x) {
+    if constexpr (R == 2) {
+        dft2<S>(x[0], x[1]);
+    } else if constexpr (R == 3) {
+        dft3<S>(x[0], x[1], x[2]);
+    } else if constexpr (R == 4) {
+        dft4<S>(x[0], x[1], x[2], x[3]);
+    } else if constexpr (R == 8) {
+        dft8<S>(x);
+    } else if constexpr (R == 13) {
+        dft13<S>(x);
+    } else {
+        static_assert(R == 16, "radix");
+        dft16<S>(x);
+    }
+}
+
+// One radix-R Stockham pass of an N-point FFT: butterfly b < N/R reads
+// in[b + (N/R) i], multiplies input i by w^(j i) (j = b mod Ns,
+// w = e^{S 2 pi i/(Ns R)} = tab[N/(Ns R)], tab[m] = e^{S 2 pi i m/N}), takes
+// the DFT-R and writes output k to out[(b / Ns) Ns R + j + Ns k].
+template <int R, int S>
+__device__ __forceinline__ void pass(const float2 *in, float2 *out, int N, int Ns, const float2 *__restrict__ tab) {
+    const int nb = N / R, stride = N / (Ns * R);
+    for (int b = threadIdx.x; b < nb; b += SEG_THREADS) {
+        const int j = b % Ns;
+        float2 x[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) x[i] = in[b + nb * i];
+        if (Ns > 1) {
+#pragma unroll
+            for (int i = 1; i < R; ++i) x[i] = cmul(x[i], tab[j * i * stride]);
+        }
+        dft_r<R, S>(x);
+        const int base = (b / Ns) * Ns * R + j;
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[base + Ns * k] = x[k];
+    }
+    __syncthreads();
+}
+
+// Radix sequences (first pass Ns = 1, no twiddles).
+template <int N, int S>
+__device__ __forceinline__ float2 *fft(float2 *a, float2 *b, const float2 *__restrict__ tab) {
+    if constexpr (N == 4992) {
+        pass<16, S>(a, b, N, 1, tab);
+        pass<8, S>(b, a, N, 16, tab);
+        pass<13, S>(a, b, N, 128, tab);
+        pass<3, S>(b, a, N, 1664, tab);
+        return a;
+    } else if constexpr (N == 1664) {
+        pass<16, S>(a, b, N, 1, tab);
+        pass<8, S>(b, a, N, 16, tab);
+        pass<13, S>(a, b, N, 128, tab);
+        return b;
+    } else if constexpr (N == 4096) {
+        pass<16, S>(a, b, N, 1, tab);
+        pass<16, S>(b, a, N, 16, tab);
+        pass<16, S>(a, b, N, 256, tab);
+        return b;
+    } else {
+        static_assert(N == 1024, "size");
+        pass<16, S>(a, b, N, 1, tab);
+        pass<8, S>(b, a, N, 16, tab);
+        pass<8, S>(a, b, N, 128, tab);
+        return b;
+    }
+}
+
+// Device tables of a preset (host-filled in FP64, rounded to FP32).
+struct Tabs {
+    const float2 *zr;     // e^{-2 pi i m/(R/2)}, m < R/2        (forward R/2-point FFT)
+    const float2 *xr;     // e^{-2 pi i k/R}, k <= R/2            (R2C post-processing)
+    const float2 *wp;     // e^{+2 pi i j/P}, j < P/2             (C2R pre-processing)
+    const float2 *zp;     // e^{+2 pi i m/(P/2)}, m < P/2         (inverse P/2-point FFT)
+};
+
+}  // namespace trim
+
+// SwarmC2R of one snapshot (workgroup = snapshot s of block bpl, 1024 threads):
+// the packed 32768-point C2R of stage A's input (DC imaginary part dropped,
+// Nyquist bin zero, LSB flip) into the block's series T[bpl][s][32768],
+// unnormalised (numpy irfft times 32768).
+template <int D>
+__global__ void __launch_bounds__(1024)
+k_trim_c2r(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb) {
+    __shared__ float lds[LDS_FFT_N];
+    __shared__ FftTw ftw;
+    fft_tw_load(&ftw, tb.fft);
+    const int bpl = blockIdx.x / D, s = blockIdx.x - bpl * D, t = threadIdx.x;
+    const GAS uint16_t *src = gptr(reinterpret_cast<const uint16_t *>(bpd[bpl].rows)) + (size_t)s * NCH;
+    const int flip = bpd[bpl].flip;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = t + 1024 * q;
+        const int km = (NCH - k) & (NCH - 1);
+        const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
+        const float kz = k == 0 ? 0.f : 1.f;
+        const uint16_t ua = src[ka], ub = src[kb];
+        const float2 a = make_float2((float)(int8_t)(ua >> 8), kz * (float)(int8_t)(ua & 0xFF));
+        const float2 b = make_float2(kz * (float)(int8_t)(ub >> 8), kz * (float)(int8_t)(ub & 0xFF));
+        const float2 sum = cadd(a, cconj(b)), dif = csub(a, cconj(b));
+        v[q] = cadd(sum, mul_si<+1>(cmul(tw<+1>(tb.n32k, (uint32_t)k), dif)));
+    }
+    fft16k<+1>(v, lds, &ftw);
+    float2 *dst = T + ((size_t)bpl * D + s) * NCH + t;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
+}
+
+// SwarmR2C + R2dbeC2R of segment g of block bpl (workgroup = segment, 256
+// threads).  Output: P samples (numpy units) at out[bpl] + g P, where out is
+// bpd[bpl].out (float samples) when F32, else stage + bpl * 128 P; FP64 sums of
+// x and x^2 of the segment at part[(bpl * 128 + g) * 2].
+template <int D, int E, bool F32>
+__global__ void __launch_bounds__(256)
+k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *__restrict__ stage,
+           double *__restrict__ part, trim::Tabs tt, int skip) {
+    constexpr int R = 256 * D, P = 256 * E, NR = R / 2, NP = P / 2;
+    constexpr int NBUF = NR > NP ? NR : NP;
+    __shared__ float2 bufa[NBUF], bufb[NBUF];
+    __shared__ double red[2][trim::SEG_THREADS / 64];
+    const int bpl = blockIdx.x / trim::NSEG, g = blockIdx.x - bpl * trim::NSEG, t = threadIdx.x;
+    const float2 *z = reinterpret_cast<const float2 *>(T + (size_t)bpl * D * 2 * NCH + (size_t)g * R);
+    for (int m = t; m < NR; m += trim::SEG_THREADS) bufa[m] = z[m];
+    __syncthreads();
+    const float2 *Z = trim::fft<NR, -1>(bufa, bufb, tt.zr);
+    float2 *W = Z == bufa ? bufb : bufa;
+    // bins skip .. skip + NP of the R2C, then the C2R pre-processing
+    auto X = [&](int k) {
+        const float2 a = Z[k == NR ? 0 : k], b = cconj(Z[(NR - k) % NR]);
+        const float2 e = cscale(cadd(a, b), 0.5f), d = cscale(csub(a, b), 0.5f);
+        return cadd(e, mul_si<-1>(cmul(tt.xr[k], d)));
+    };
+    for (int j = t; j < NP; j += trim::SEG_THREADS) {
+        float2 y = X(skip + j), ym = X(skip + NP - j);
+        if (j == 0) {
+            y.y = 0.f;         // Y[0] and Y[P/2] enter with their real parts only
+            ym.y = 0.f;
+        }
+        const float2 sum = cadd(y, cconj(ym)), dif = csub(y, cconj(ym));
+        W[j] = cadd(sum, mul_si<+1>(cmul(tt.wp[j], dif)));
+    }
+    __syncthreads();
+    const float2 *w = trim::fft<NP, +1>(W, W == bufa ? bufb : bufa, tt.zp);
+    // samples (2m, 2m+1) = (Re, Im) w[m] / (32768 P)
+    const float sc = 1.0f / (32768.0f * (float)P);
+    float2 *dst = reinterpret_cast<float2 *>((F32 ? (float *)bpd[bpl].out : stage + (size_t)bpl * trim::NSEG * P) +
+                                             (size_t)g * P);
+    double s1 = 0.0, s2 = 0.0;
+    for (int m = t; m < NP; m += trim::SEG_THREADS) {
+        const float2 x = cscale(w[m], sc);
+        dst[m] = x;
+        s1 += (double)x.x + (double)x.y;
+        s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+    }
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = s1;
+        red[1][t >> 6] = s2;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < trim::SEG_THREADS / 64; ++i) {
+            a += red[0][i];
+            b += red[1][i];
+        }
+        part[((size_t)bpl * trim::NSEG + g) * 2] = a;
+        part[((size_t)bpl * trim::NSEG + g) * 2 + 1] = b;
+    }
+}
+
+// Mean and sigma (ddof 1) of each block from the segment sums: ms[b] = (mean, sigma).
+__global__ void __launch_bounds__(128)
+k_trim_stats(const double *__restrict__ part, int P, int bp0, double2 *__restrict__ ms, double *__restrict__ stats_out) {
+    __shared__ double red[2][trim::NSEG];
+    const int t = threadIdx.x, b = blockIdx.x;
+    red[0][t] = part[((size_t)b * trim::NSEG + t) * 2];
+    red[1][t] = part[((size_t)b * trim::NSEG + t) * 2 + 1];
+    __syncthreads();
+    for (int s = trim::NSEG / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            red[0][t] += red[0][t + s];
+            red[1][t] += red[1][t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double L = (double)trim::NSEG * P;
+        const double mean = red[0][0] / L;
+        double var = (red[1][0] - L * mean * mean) / (L - 1.0);
+        if (var < 0.0) var = 0.0;
+        const double sig = sqrt(var);
+        ms[b] = make_double2(mean, sig);
+        if (stats_out) {
+            stats_out[(size_t)(bp0 + b) * 2] = mean;
+            stats_out[(size_t)(bp0 + b) * 2 + 1] = sig;
+        }
+    }
+}
+
+// 2-bit codes of block bpl in natural sample order (vdif_inout_gpu_thread.cu:565-583:
+// v = x - off; ((|v| >= th) | 2) ^ 3 (v < 0); sample i of a word at bits 2i),
+// stream byte q to frame q / 8192 of the block's output.  Thread = one 32-bit word.
+__global__ void __launch_bounds__(256)
+k_trim_pack(const float *__restrict__ stage, const float2 *__restrict__ thr, const BpDesc *__restrict__ bpd, int P,
+            OutDesc od) {
+    const int words = trim::NSEG * P / 16;
+    const int wpb = words / 256;                      // workgroups per block
+    const int bpl = blockIdx.x / wpb, wi = (blockIdx.x - bpl * wpb) * 256 + threadIdx.x;
+    const float4 *src = reinterpret_cast<const float4 *>(stage + (size_t)bpl * trim::NSEG * P) + (size_t)wi * 4;
+    const float2 to = thr[bpl];
+    uint32_t word = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float4 f = src[i];
+        const float x[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v = x[j] - to.y;
+            const uint32_t c = ((((fabsf(v) >= to.x) ? 1u : 0u) | 2u) ^ (v < 0.f ? 3u : 0u)) & 3u;
+            word |= c << (2 * (4 * i + j));
+        }
+    }
+    const int64_t q = (int64_t)wi * 4;
+    uint8_t *dst = bpd[bpl].out + (q / OUT_FRAME_PAYLOAD) * od.frame_bytes + od.hdr_bytes + (q % OUT_FRAME_PAYLOAD);
+    *reinterpret_cast<uint32_t *>(dst) = word;
+}

@@ -32,7 +32,8 @@ def station_code(s):
 class SdbeStage:
     def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=256, device=0,
                  station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0), use_graph=False,
-                 c1_tiles_per_wg=0, max_batch=0, fft_precision=L.FFT_FP32):
+                 c1_tiles_per_wg=0, max_batch=0, fft_precision=L.FFT_FP32, rate_frac=L.RATE_64_39,
+                 skip_mhz=0):
         self.lib = L.load()
         cfg = L.Config()
         L.check(self.lib.sdbe_default_config(C.byref(cfg)), "sdbe_default_config")
@@ -49,7 +50,17 @@ class SdbeStage:
         cfg.c1_tiles_per_wg = c1_tiles_per_wg
         cfg.max_batch_blockpols = max_batch
         cfg.fft_precision = fft_precision
+        cfg.rate_frac = rate_frac
+        cfg.skip_mhz = skip_mhz
         self.cfg = cfg
+        # geometry of a block-pol: snapshots in, samples and frames out
+        # (include/sdbe_stage.h, SDBE_RATE_*)
+        if rate_frac == L.RATE_64_39:
+            self.snaps, self.samples, self.frames = 39, L.SAMPLES_PER_BLOCK, L.OUT_FRAMES_PER_BLOCK
+        else:
+            d, e = {L.RATE_6_11: (39, 32), L.RATE_8_11: (13, 8)}.get(rate_frac, (0, 0))
+            self.snaps, self.samples, self.frames = d, 128 * 256 * e, e
+        self.frames_per_group = (L.SNAPSHOTS_PER_GROUP // self.snaps) * self.frames if self.snaps else 0
         self.device = device
         self.out_mode = out_mode
         self.frame_bytes = L.OUT_PAYLOAD_BYTES + (L.OUT_HEADER_BYTES if out_mode == L.OUT_FRAMES else 0)
@@ -103,8 +114,9 @@ class SdbeStage:
 
     # -------------------------------------------------------------- buffers
     def group_out(self):
-        """Output buffer of one group, [2 pols][8192 frames][frame bytes] uint8."""
-        return torch.empty((2, L.OUT_FRAMES_PER_GROUP, self.frame_bytes), dtype=torch.uint8,
+        """Output buffer of one group, [2 pols][frames per group][frame bytes] uint8
+        (8192 frames for the 64/39 resampler, 4096 for 6/11, 3072 for 8/11)."""
+        return torch.empty((2, self.frames_per_group, self.frame_bytes), dtype=torch.uint8,
                            device="cuda:%d" % self.device)
 
     @staticmethod
@@ -116,7 +128,7 @@ class SdbeStage:
     def process_group(self, d_pkts, n_pkts, b_zero, out=None, bitmap=None, stream=None):
         if out is None:
             out = self.group_out()
-        assert out.is_contiguous() and out.numel() >= 2 * L.OUT_FRAMES_PER_GROUP * self.frame_bytes
+        assert out.is_contiguous() and out.numel() >= 2 * self.frames_per_group * self.frame_bytes
         assert d_pkts.numel() >= n_pkts * L.PKT_BYTES
         stride = out.numel() // 2
         rc = self.lib.sdbe_process_group(self.h, _ptr(d_pkts), n_pkts, b_zero, _ptr(out), stride,
@@ -136,14 +148,13 @@ class SdbeStage:
         return b0, b1, bm
 
     def process_spectra(self, d_beng, out=None, stream=None):
-        """d_beng: int8 [n_bp][39][16384][2 = (im, re)] on the device."""
+        """d_beng: int8 [n_bp][snapshots per block][16384][2 = (im, re)] on the device."""
         n_bp = d_beng.shape[0]
         assert d_beng.dtype == torch.int8 and d_beng.is_contiguous()
-        assert tuple(d_beng.shape[1:]) == (39, L.CHANNELS, 2)
+        assert tuple(d_beng.shape[1:]) == (self.snaps, L.CHANNELS, 2)
         if out is None:
-            out = torch.empty((n_bp, L.OUT_FRAMES_PER_BLOCK, self.frame_bytes), dtype=torch.uint8,
-                              device=d_beng.device)
-        assert out.numel() >= n_bp * L.OUT_FRAMES_PER_BLOCK * self.frame_bytes
+            out = torch.empty((n_bp, self.frames, self.frame_bytes), dtype=torch.uint8, device=d_beng.device)
+        assert out.numel() >= n_bp * self.frames * self.frame_bytes
         rc = self.lib.sdbe_process_spectra(self.h, _ptr(d_beng), n_bp, _ptr(out), _stream(stream))
         L.check(rc, "sdbe_process_spectra", self.h)
         return out
@@ -151,8 +162,8 @@ class SdbeStage:
     def resample_f32(self, d_beng, stream=None):
         n_bp = d_beng.shape[0]
         assert d_beng.dtype == torch.int8 and d_beng.is_contiguous()
-        assert tuple(d_beng.shape[1:]) == (39, L.CHANNELS, 2)
-        x = torch.empty((n_bp, L.SAMPLES_PER_BLOCK), dtype=torch.float32, device=d_beng.device)
+        assert tuple(d_beng.shape[1:]) == (self.snaps, L.CHANNELS, 2)
+        x = torch.empty((n_bp, self.samples), dtype=torch.float32, device=d_beng.device)
         st = torch.empty((n_bp, 2), dtype=torch.float64, device=d_beng.device)
         rc = self.lib.sdbe_resample_spectra_f32(self.h, _ptr(d_beng), n_bp, _ptr(x), _ptr(st),
                                                 _stream(stream))

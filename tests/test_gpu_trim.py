@@ -1,0 +1,169 @@
+"""GPU parity of the band-trimmed resampler of the aphids GPU thread
+(SDBE_RATE_6_11 and SDBE_RATE_8_11; vdif_inout_gpu_thread.cu:589-709, plans
+:269-333, presets sample_rates.h:21-40) against the FP64 oracle
+(oracle.resample_trimmed, pinned in tests/test_oracle_trim.py).
+
+Bars, as for the FP32 64/39 pipeline:
+* float samples: RMS <= 2e-6 sigma and max <= 5e-5 sigma of the FP64 chain;
+  block mean and sigma within 1e-6 (relative to sigma);
+* 2-bit codes: every code that differs from the oracle's lies within 1e-5 sigma
+  of a threshold; headers bit-exact (frames of 8224 bytes, E per block-pol);
+* a whole group through sdbe_process_group (shuffled packets with drops,
+  REFERENCE latch): the same code bar on blocks of both polarisations.
+Rejected: unknown presets, a skip past the band, precision variants, ragged
+batches and re-framed output in a band-trimmed handle.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sdbe_oracle as O
+from sdbe_amd import synth as S
+
+pytestmark = pytest.mark.gpu
+
+BAND_SIGMA = 1e-5
+NCH = 16384
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _stage(frac, skip=150, **kw):
+    from sdbe_amd import _lib as L
+    from sdbe_amd.stage import SdbeStage
+    kw.setdefault("policy", L.THRESH_PER_BLOCK)
+    kw.setdefault("out_mode", L.OUT_PAYLOAD)
+    kw.setdefault("chunk", 2)
+    return SdbeStage(rate_frac=frac, skip_mhz=skip, **kw)
+
+
+def _spectra(seed, D, n):
+    rng = S.rng_for(seed)
+    codes = S.gaussian_codes(rng, D * n)
+    return S.codes_to_beng_int8(codes).reshape(n, D, NCH, 2)
+
+
+def _oracle_block(beng_block, E, skip):
+    return O.resample_trimmed(O.inflate(beng_block)[:, :NCH], E, skip)
+
+
+def _code_report(x64, codes_gpu, th, off):
+    c64 = O.quantize2bit(x64, th, off)
+    bad = np.nonzero(c64 != codes_gpu)[0]
+    if bad.size == 0:
+        return 0, 0
+    v = x64[bad] - off
+    dist = np.min(np.abs(np.stack([v - th, v, v + th])), axis=0) / th
+    return int(bad.size), int((dist > BAND_SIGMA).sum())
+
+
+@pytest.mark.parametrize("frac", [6, 8])
+def test_trim_samples_vs_fp64(torch_cuda, frac):
+    torch = torch_cuda
+    D, E, R, P, skip = O.trim_geometry(frac, 150)
+    n = 2
+    beng = _spectra(600 + frac, D, n)
+    st = _stage(frac, max_spectra=n)
+    _say("[trim] %d/11 resample_f32 n=%d (D=%d E=%d skip=%d)" % (frac, n, D, E, skip))
+    x, stats = st.resample_f32(torch.from_numpy(beng).cuda())
+    x = x.cpu().numpy().astype(np.float64)
+    stats = stats.cpu().numpy()
+    assert x.shape == (n, 128 * P)
+    for b in range(n):
+        x64 = _oracle_block(beng[b], E, skip)
+        mean, sig = O.block_stats(x64)
+        err = x[b] - x64
+        rms, mx = np.sqrt(np.mean(err ** 2)) / sig, np.abs(err).max() / sig
+        _say("[trim] %d/11 block %d: RMS %.3e sigma, max %.3e sigma; mean %.6g sigma %.6g (oracle %.6g %.6g)"
+             % (frac, b, rms, mx, stats[b, 0], stats[b, 1], mean, sig))
+        assert rms <= 2e-6 and mx <= 5e-5
+        assert abs(stats[b, 0] - mean) <= 1e-6 * sig and abs(stats[b, 1] - sig) <= 1e-6 * sig
+    st.close()
+
+
+@pytest.mark.parametrize("frac", [6, 8])
+def test_trim_codes_and_headers(torch_cuda, frac):
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    D, E, R, P, skip = O.trim_geometry(frac, 102)
+    n = 2
+    beng = _spectra(700 + frac, D, n)
+    st = _stage(frac, skip=102, out_mode=L.OUT_FRAMES, max_spectra=n, station="Sm", bdc=1, rx=1)
+    st.set_state(secs=4321, df=124995, ref_epoch=31, psn=900)
+    _say("[trim] %d/11 process_spectra frames n=%d" % (frac, n))
+    out = st.process_spectra(torch.from_numpy(beng).cuda()).cpu().numpy()
+    assert out.shape == (n, E, 8224)
+    hdr = out[:, :, :32].copy().view(np.uint32).reshape(n * E, 8)
+    pay = out[:, :, 32:].copy().view(np.uint32).reshape(n, -1)
+    state = O.StreamState(secs=4321, df=124995, ref_epoch=31, psn=900, station=O.station_from_str("Sm"),
+                          bdc=1, rx=1)
+    ref_hdr, ref_next = O.stamp_headers(n * E, state)
+    assert np.array_equal(hdr, ref_hdr[0])
+    assert st.get_state() == dict(secs=ref_next.secs, df=ref_next.df, ref_epoch=31, psn=ref_next.psn)
+    for b in range(n):
+        x64 = _oracle_block(beng[b], E, skip)
+        mean, sig = O.block_stats(x64)
+        codes = O.unpack_codes(pay[b])
+        n_bad, n_out = _code_report(x64, codes, sig, mean)
+        _say("[trim] %d/11 block %d: %d codes differ, %d outside %.0e sigma" % (frac, b, n_bad, n_out, BAND_SIGMA))
+        assert codes.size == 128 * P
+        assert n_out == 0
+    st.close()
+
+
+@pytest.mark.parametrize("frac", [6, 8])
+def test_trim_group_reference_policy(torch_cuda, frac):
+    from sdbe_amd import _lib as L
+    D, E, R, P, skip = O.trim_geometry(frac, 150)
+    rng = S.rng_for(800 + frac)
+    c0 = S.gaussian_codes(rng, 39 * 128)
+    c1 = S.gaussian_codes(rng, 39 * 128)
+    b0 = 4242
+    pk = S.packetize(c0, c1, b0)
+    pk_shuf, _ = S.shuffle_and_drop(pk, rng, drop_frac=0.001)
+    st = _stage(frac, policy=L.THRESH_REFERENCE, chunk=256)
+    d = st.to_device_packets(pk_shuf)
+    _say("[trim] %d/11 process_group" % frac)
+    out = st.process_group(d, pk_shuf.shape[0], b0).cpu().numpy()
+    nblk = 4992 // D
+    assert out.shape == (2, nblk * E, 8192)
+    ref = O.vdif_to_beng(pk_shuf, b0)
+    snaps = [ref.beng[p].reshape(-1, NCH, 2) for p in range(2)]
+    x00 = _oracle_block(snaps[0][0:D], E, skip)
+    mean0, sig0 = O.block_stats(x00)            # REFERENCE: pol 0 block 0 latches both
+    for blk in (0, 1, nblk - 1):
+        for p in range(2):
+            x64 = x00 if (blk, p) == (0, 0) else _oracle_block(snaps[p][blk * D:(blk + 1) * D], E, skip)
+            words = out[p, blk * E:(blk + 1) * E].copy().view(np.uint32).reshape(-1)
+            n_bad, n_out = _code_report(x64, O.unpack_codes(words), sig0, mean0)
+            _say("[trim] %d/11 group block %d pol %d: %d codes differ, %d outside band" % (frac, blk, p, n_bad, n_out))
+            assert n_out == 0
+    st.close()
+
+
+def test_trim_rejects(torch_cuda):
+    from sdbe_amd import _lib as L
+    from sdbe_amd.stage import SdbeStage
+    for kw in (dict(rate_frac=7), dict(rate_frac=6, skip_mhz=300), dict(rate_frac=8, skip_mhz=700),
+               dict(rate_frac=6, skip_mhz=-1), dict(rate_frac=6, fft_precision=L.FFT_FP16)):
+        with pytest.raises(L.SdbeError):
+            SdbeStage(**kw)
+    st = _stage(6)
+    lib = st.lib
+    assert lib.sdbe_set_output_alignment(st.h, 1, 0, 0) == L.SDBE_ERR_STATE
+    desc = L.StreamDesc()
+    assert lib.sdbe_process_blocks(st.h, C.byref(desc), 1, None) == L.SDBE_ERR_STATE
+    st.close()

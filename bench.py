@@ -116,6 +116,10 @@ def main():
     ap.add_argument("--policy", default="per_block", choices=["per_block", "reference"])
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1, 2, 3],
                     help="FFT arithmetic: 0 FP32 (default), 1 BF16X3, 2 BF16, 3 FP16 matrix-core passes")
+    ap.add_argument("--rate", type=int, default=0, choices=[0, 6, 8],
+                    help="resampler: 0 the 64/39 path (default), 6 / 8 the aphids band-trimmed path "
+                         "of the 6/11 or 8/11 SWARM rate")
+    ap.add_argument("--skip-mhz", type=int, default=150, help="band-trimmed path: MHz skipped")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--copy-probe", action="store_true",
@@ -152,7 +156,19 @@ def main():
     n_pkts = sum(pk.shape[0] for pk, _ in groups)
     policy = L.THRESH_PER_BLOCK if args.policy == "per_block" else L.THRESH_REFERENCE
     st = SdbeStage(policy=policy, out_mode=L.OUT_FRAMES, chunk=args.chunk, station="Sm",
-                   fft_precision=args.precision)
+                   fft_precision=args.precision, rate_frac=args.rate, skip_mhz=args.skip_mhz)
+    alg_bytes = dict(ALG_BYTES)
+    swarm_gsps = 2.496
+    if args.rate:
+        # band-trimmed kernels, per block of D snapshots: int8 rows -> series ->
+        # 128 segments of P samples -> 2-bit payload
+        d, e = {6: (39, 32), 8: (13, 8)}[args.rate]
+        p = 256 * e
+        alg_bytes.update({"k_trim_c2r": d * 16384 * 2 + d * 32768 * 4,
+                          "k_trim_seg": d * 32768 * 4 + 128 * p * 4,
+                          "k_trim_pack": 128 * p * 4 + 128 * p // 4,
+                          "k_headers": e * 32, "k_stats": 128 * 16})
+        swarm_gsps = 4.576 * args.rate / 11
     st.set_state(secs=1000, df=0, ref_epoch=30, psn=1)
     dpk = [st.to_device_packets(pk) for pk, _ in groups]
     outs = [st.group_out() for _ in groups]
@@ -179,14 +195,14 @@ def main():
     st.profile(False)
     stats = st.stats()
 
-    n_bp = 2 * 2 * 128
+    n_bp = 2 * 2 * (4992 // st.snaps)
     ms_step = (t1 - t0) * 1e3 / args.steps
-    value = n_bp * SAMPLES_PER_BP / (ms_step * 1e-3) / 1e9
+    value = 2 * 2 * 4992 * 32768 / (ms_step * 1e-3) / 1e9     # input samples, any resampler
     bp_per_s = n_bp / (ms_step * 1e-3)
 
     kernels = {}
     for name, (launches, total_ms, units) in prof.items():
-        ab = ALG_BYTES.get(name)
+        ab = alg_bytes.get(name)
         kernels[name] = {
             "launches": launches,
             "avg_us": round(1e3 * total_ms / launches, 3),
@@ -195,7 +211,7 @@ def main():
         }
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     dname, (dl, dms, du) = dom
-    d_bytes_per_launch = ALG_BYTES[dname] * du / dl
+    d_bytes_per_launch = alg_bytes[dname] * du / dl
     d_avg_s = dms * 1e-3 / dl
     achieved = d_bytes_per_launch / d_avg_s / 1e9
 
@@ -213,8 +229,12 @@ def main():
         "dtype": ["f32", "f32+bf16x3-mfma", "f32+bf16-mfma", "f32+f16-mfma"][args.precision],
         "data": "synthetic",
         "config": {
-            "workload": "plain FP32 pipeline at full rate: 256 blocks x 2 pols x 16384 ch "
-                        "(2 groups of 39 B-frames), shuffled packets, 0.1% dropped, VDIF frames out",
+            "workload": ("plain FP32 pipeline at full rate: 256 blocks x 2 pols x 16384 ch "
+                         "(2 groups of 39 B-frames), shuffled packets, 0.1% dropped, VDIF frames out"
+                         if not args.rate else
+                         "band-trimmed %d/11 path (skip %d MHz): 2 groups of 39 B-frames x 2 pols x 16384 ch, "
+                         "shuffled packets, 0.1%% dropped, VDIF frames out" % (args.rate, args.skip_mhz)),
+            "rate": ["64/39", None, None, None, None, None, "6/11 trimmed", None, "8/11 trimmed"][args.rate],
             "block_pols_per_step": n_bp,
             "packets_per_step": int(n_pkts),
             "threshold_policy": args.policy,
@@ -233,14 +253,16 @@ def main():
         },
         "pipeline": {
             "block_pols_per_s": round(bp_per_s, 1),
-            "b_alg_frac": round(B_ALG_PER_BP * bp_per_s / HBM_PEAK, 4),
-            "compulsory_frac": round(COMPULSORY_PER_BP * bp_per_s / HBM_PEAK, 5),
-            "realtime_pol_streams": round(value / 2.496, 2),
+            "b_alg_frac": round(B_ALG_PER_BP * bp_per_s / HBM_PEAK, 4) if not args.rate else None,
+            "compulsory_frac": round(COMPULSORY_PER_BP * bp_per_s / HBM_PEAK, 5) if not args.rate else None,
+            "realtime_pol_streams": round(value / swarm_gsps, 2),
         },
         "kernels": kernels,
         "stage_stats": {k: stats[k] for k in ("n_unique", "n_missing", "n_duplicate", "n_invalid")},
     }
-    if not args.no_cpu:
+    if args.rate:
+        rec["cpu_baseline"] = None       # the CPU leg times the 64/39 chain only
+    elif not args.no_cpu:
         log("[bench] cpu baseline")
         rec["cpu_baseline"] = cpu_baseline(groups)
     print(json.dumps(rec))

@@ -54,6 +54,13 @@ inline bool geometry(int rate_frac, int skip_mhz, Geometry &g) {
     return skip_mhz >= 0 && g.skip + g.P / 2 <= g.R / 2;
 }
 
+// LDS position of element e of a segment buffer: an XOR swizzle within blocks
+// of 16 float2 (one 128-byte bank row).  The Stockham writes of a pass with
+// Ns = 1 (out[b R + k]) and Ns = 16 would otherwise put 16 lanes on one bank
+// group; with the swizzle every read and write of the passes spreads over the
+// 16 groups (N is a multiple of 16 for every size here).
+__device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
+
 // DFT of radix R in place, natural order in and out, sign S.
 template <int R, int S>
 __device__ __forceinline__ void dft_r(float2 *x) {
@@ -77,51 +84,76 @@ __device__ __forceinline__ void dft_r(float2 *x) {
 // in[b + (N/R) i], multiplies input i by w^(j i) (j = b mod Ns,
 // w = e^{S 2 pi i/(Ns R)} = tab[N/(Ns R)], tab[m] = e^{S 2 pi i m/N}), takes
 // the DFT-R and writes output k to out[(b / Ns) Ns R + j + Ns k].
-template <int R, int S>
-__device__ __forceinline__ void pass(const float2 *in, float2 *out, int N, int Ns, const float2 *__restrict__ tab) {
-    const int nb = N / R, stride = N / (Ns * R);
-    for (int b = threadIdx.x; b < nb; b += SEG_THREADS) {
-        const int j = b % Ns;
-        float2 x[R];
+// Sizes are compile-time, so the butterfly loop unrolls and every load of a
+// thread (LDS data, and the table entries w, w^2, w^4, w^8 from which the other
+// powers are products of at most four) is issued before the first is used.
+// Loads of a partial last iteration read butterfly N/R - 1; only its writes are
+// guarded.  In place: all reads of the pass happen before the barrier that
+// precedes the first write.
+template <int N, int R, int NS, int S>
+__device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tab) {
+    const float2 *in = buf;
+    float2 *out = buf;
+    constexpr int NB = N / R, IT = (NB + SEG_THREADS - 1) / SEG_THREADS, STRIDE = N / (NS * R);
+    float2 x[IT][R];
 #pragma unroll
-        for (int i = 0; i < R; ++i) x[i] = in[b + nb * i];
-        if (Ns > 1) {
+    for (int it = 0; it < IT; ++it) {
+        const int b0 = threadIdx.x + it * SEG_THREADS;
+        const int b = (NB % SEG_THREADS == 0 || it < IT - 1) ? b0 : (b0 < NB ? b0 : NB - 1);
 #pragma unroll
-            for (int i = 1; i < R; ++i) x[i] = cmul(x[i], tab[j * i * stride]);
+        for (int i = 0; i < R; ++i) x[it][i] = in[swz16(b + NB * i)];
+        if constexpr (NS > 1) {
+            const int m = (b % NS) * STRIDE;
+            float2 pw[16];
+            pw[1] = tab[m];
+            if constexpr (R > 2) pw[2] = tab[2 * m];
+            if constexpr (R > 4) pw[4] = tab[4 * m];
+            if constexpr (R > 8) pw[8] = tab[8 * m];
+#pragma unroll
+            for (int i = 3; i < R; ++i) {
+                if (i == 4 || i == 8) continue;
+                const int hi = i >= 8 ? 8 : (i >= 4 ? 4 : 2);
+                pw[i] = cmul(pw[hi], pw[i - hi]);
+            }
+#pragma unroll
+            for (int i = 1; i < R; ++i) x[it][i] = cmul(x[it][i], pw[i]);
         }
-        dft_r<R, S>(x);
-        const int base = (b / Ns) * Ns * R + j;
+    }
+    __syncthreads();   // every read of the pass is done
 #pragma unroll
-        for (int k = 0; k < R; ++k) out[base + Ns * k] = x[k];
+    for (int it = 0; it < IT; ++it) {
+        const int b = threadIdx.x + it * SEG_THREADS;
+        dft_r<R, S>(x[it]);
+        if (NB % SEG_THREADS == 0 || it < IT - 1 || b < NB) {
+            const int base = (b / NS) * NS * R + (b % NS);
+#pragma unroll
+            for (int k = 0; k < R; ++k) out[swz16(base + NS * k)] = x[it][k];
+        }
     }
     __syncthreads();
 }
 
-// Radix sequences (first pass Ns = 1, no twiddles).
+// Radix sequences (first pass Ns = 1, no twiddles), in place.
 template <int N, int S>
-__device__ __forceinline__ float2 *fft(float2 *a, float2 *b, const float2 *__restrict__ tab) {
+__device__ __forceinline__ void fft(float2 *a, const float2 *__restrict__ tab) {
     if constexpr (N == 4992) {
-        pass<16, S>(a, b, N, 1, tab);
-        pass<8, S>(b, a, N, 16, tab);
-        pass<13, S>(a, b, N, 128, tab);
-        pass<3, S>(b, a, N, 1664, tab);
-        return a;
+        pass<N, 16, 1, S>(a, tab);
+        pass<N, 8, 16, S>(a, tab);
+        pass<N, 13, 128, S>(a, tab);
+        pass<N, 3, 1664, S>(a, tab);
     } else if constexpr (N == 1664) {
-        pass<16, S>(a, b, N, 1, tab);
-        pass<8, S>(b, a, N, 16, tab);
-        pass<13, S>(a, b, N, 128, tab);
-        return b;
+        pass<N, 16, 1, S>(a, tab);
+        pass<N, 8, 16, S>(a, tab);
+        pass<N, 13, 128, S>(a, tab);
     } else if constexpr (N == 4096) {
-        pass<16, S>(a, b, N, 1, tab);
-        pass<16, S>(b, a, N, 16, tab);
-        pass<16, S>(a, b, N, 256, tab);
-        return b;
+        pass<N, 16, 1, S>(a, tab);
+        pass<N, 16, 16, S>(a, tab);
+        pass<N, 16, 256, S>(a, tab);
     } else {
         static_assert(N == 1024, "size");
-        pass<16, S>(a, b, N, 1, tab);
-        pass<8, S>(b, a, N, 16, tab);
-        pass<8, S>(a, b, N, 128, tab);
-        return b;
+        pass<N, 16, 1, S>(a, tab);
+        pass<N, 8, 16, S>(a, tab);
+        pass<N, 8, 128, S>(a, tab);
     }
 }
 
@@ -138,7 +170,8 @@ struct Tabs {
 // SwarmC2R of one snapshot (workgroup = snapshot s of block bpl, 1024 threads):
 // the packed 32768-point C2R of stage A's input (DC imaginary part dropped,
 // Nyquist bin zero, LSB flip) into the block's series T[bpl][s][32768],
-// unnormalised (numpy irfft times 32768).
+// unnormalised (numpy irfft times 32768).  (A persistent form with the next
+// snapshot's int8 pairs prefetched during the last pass measured 8% slower.)
 template <int D>
 __global__ void __launch_bounds__(1024)
 k_trim_c2r(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb) {
@@ -177,38 +210,49 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
            double *__restrict__ part, trim::Tabs tt, int skip) {
     constexpr int R = 256 * D, P = 256 * E, NR = R / 2, NP = P / 2;
     constexpr int NBUF = NR > NP ? NR : NP;
-    __shared__ float2 bufa[NBUF], bufb[NBUF];
+    // one in-place buffer: 39 KB at 6/11, four workgroups per CU
+    __shared__ float2 buf[NBUF];
     __shared__ double red[2][trim::SEG_THREADS / 64];
     const int bpl = blockIdx.x / trim::NSEG, g = blockIdx.x - bpl * trim::NSEG, t = threadIdx.x;
     const float2 *z = reinterpret_cast<const float2 *>(T + (size_t)bpl * D * 2 * NCH + (size_t)g * R);
-    for (int m = t; m < NR; m += trim::SEG_THREADS) bufa[m] = z[m];
+    for (int m = t; m < NR; m += trim::SEG_THREADS) buf[trim::swz16(m)] = z[m];
     __syncthreads();
-    const float2 *Z = trim::fft<NR, -1>(bufa, bufb, tt.zr);
-    float2 *W = Z == bufa ? bufb : bufa;
-    // bins skip .. skip + NP of the R2C, then the C2R pre-processing
+    trim::fft<NR, -1>(buf, tt.zr);
+    // bins skip .. skip + NP of the R2C, then the C2R pre-processing; the NP / 256
+    // values of a thread stay in registers until every read of Z is done
     auto X = [&](int k) {
-        const float2 a = Z[k == NR ? 0 : k], b = cconj(Z[(NR - k) % NR]);
+        const float2 a = buf[trim::swz16(k == NR ? 0 : k)], b = cconj(buf[trim::swz16((NR - k) % NR)]);
         const float2 e = cscale(cadd(a, b), 0.5f), d = cscale(csub(a, b), 0.5f);
         return cadd(e, mul_si<-1>(cmul(tt.xr[k], d)));
     };
-    for (int j = t; j < NP; j += trim::SEG_THREADS) {
+    constexpr int JT = NP / trim::SEG_THREADS;
+    static_assert(NP % trim::SEG_THREADS == 0, "C2R size");
+    float2 wv[JT];
+#pragma unroll
+    for (int q = 0; q < JT; ++q) {
+        const int j = t + trim::SEG_THREADS * q;
         float2 y = X(skip + j), ym = X(skip + NP - j);
         if (j == 0) {
             y.y = 0.f;         // Y[0] and Y[P/2] enter with their real parts only
             ym.y = 0.f;
         }
         const float2 sum = cadd(y, cconj(ym)), dif = csub(y, cconj(ym));
-        W[j] = cadd(sum, mul_si<+1>(cmul(tt.wp[j], dif)));
+        wv[q] = cadd(sum, mul_si<+1>(cmul(tt.wp[j], dif)));
     }
     __syncthreads();
-    const float2 *w = trim::fft<NP, +1>(W, W == bufa ? bufb : bufa, tt.zp);
+#pragma unroll
+    for (int q = 0; q < JT; ++q) buf[trim::swz16(t + trim::SEG_THREADS * q)] = wv[q];
+    __syncthreads();
+    trim::fft<NP, +1>(buf, tt.zp);
+    const float2 *w = buf;
     // samples (2m, 2m+1) = (Re, Im) w[m] / (32768 P)
     const float sc = 1.0f / (32768.0f * (float)P);
     float2 *dst = reinterpret_cast<float2 *>((F32 ? (float *)bpd[bpl].out : stage + (size_t)bpl * trim::NSEG * P) +
                                              (size_t)g * P);
     double s1 = 0.0, s2 = 0.0;
+#pragma unroll 4
     for (int m = t; m < NP; m += trim::SEG_THREADS) {
-        const float2 x = cscale(w[m], sc);
+        const float2 x = cscale(w[trim::swz16(m)], sc);
         dst[m] = x;
         s1 += (double)x.x + (double)x.y;
         s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;

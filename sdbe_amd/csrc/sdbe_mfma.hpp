@@ -52,8 +52,8 @@ template <int S, int P>
 struct MfmaW {
     bf16x8 re_hi, re_lo, im_hi, im_lo;
     f16x8 re_h, im_h;
-    __device__ __forceinline__ void init() {
-        const int l = threadIdx.x & 63, k = l & 15, kb = l >> 4;
+    __device__ __forceinline__ void init(int t) {
+        const int l = t & 63, k = l & 15, kb = l >> 4;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int r = 4 * kb + (e >> 1);
@@ -125,8 +125,9 @@ __device__ __forceinline__ void mfma_dft16(float2 *x, const MfmaW<S, P> &W) {
 
 // Element index of register 4 g + j of this lane in the MFMA layout of a pass
 // input: DFT d = 64 w + 16 g + n, point r = 4 kb + j, element d + 1024 r.
-__device__ __forceinline__ int mfma_elem(int g, int j) {
-    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+// t is threadIdx.x (a persistent caller passes an opaque copy, see row_tid).
+__device__ __forceinline__ int mfma_elem(int g, int j, int t) {
+    const int w = t >> 6, l = t & 63;
     return 64 * w + 16 * g + (l & 15) + 1024 * (4 * (l >> 4) + j);
 }
 
@@ -135,8 +136,8 @@ __device__ __forceinline__ int mfma_elem(int g, int j) {
 // on return v[q] = element threadIdx.x + 1024 q of the transform.
 template <int S, int P>
 __device__ __forceinline__ void fft16k_mfma(float2 *v, float *lds, const FftTw *tw,
-                                            const MfmaW<S, P> &W) {
-    const int t = threadIdx.x, w = t >> 6, l = t & 63, n = l & 15, kb = l >> 4;
+                                            const MfmaW<S, P> &W, int t) {
+    const int w = t >> 6, l = t & 63, n = l & 15, kb = l >> 4;
 #pragma unroll
     for (int pass = 0; pass < 3; ++pass) {
         const int ns = pass == 0 ? 1 : (pass == 1 ? 16 : 256);
@@ -184,7 +185,7 @@ __device__ __forceinline__ void fft16k_mfma(float2 *v, float *lds, const FftTw *
             for (int g = 0; g < 4; ++g)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    v[4 * g + j] = lds_get(lds, lds_re(mfma_elem(g, j)));
+                    v[4 * g + j] = lds_get(lds, lds_re(mfma_elem(g, j, t)));
                 }
         }
     }

@@ -272,7 +272,7 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
     fft_tw_load(&ftw, tb.fft);
     int lin = blockIdx.x;
     float2 nx[16];
-    {
+    if constexpr (P == 0) {
         int bpl, k1;
         b_row(lin, xcd_map, bpl, k1);
         b_load(V, bpl, k1, nx);
@@ -309,20 +309,19 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
         } else {
             // the MFMA path runs both transforms with the inverse-sign operands: the
             // forward FFT is conj(IFFT(conj(x)))
+            // the matrix-core variants hold their operands across the passes and
+            // load their own row here, straight into the matrix-core layout
+            // (no registers to spare for a prefetch)
             MfmaW<+1, P> W;
-            W.init();
+            W.init(t);
             {
-                const int rd = lds_re(t);
-                __syncthreads();   // the previous row's LDS reads are done
-#pragma unroll
-                for (int q = 0; q < 16; ++q) lds_put(lds, rd + 2048 * q, nx[q]);
-                __syncthreads();
+                const float2 *src = V + ((size_t)bpl * NBLK + k1) * NCH;
 #pragma unroll
                 for (int g = 0; g < 4; ++g)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) v[4 * g + j] = lds_get(lds, lds_re(mfma_elem(g, j)));
+                    for (int j = 0; j < 4; ++j) v[4 * g + j] = src[mfma_elem(g, j, t)];
             }
-            fft16k_mfma<+1, P>(v, lds, &ftw, W);
+            fft16k_mfma<+1, P>(v, lds, &ftw, W, t);
             // natural order -> MFMA layout through LDS; the twiddle of element
             // e = d + 1024 r is applied there: e^{-2 pi i e k1 / M'} = A(d) * B(r)
             const int rd = lds_re(t), kb = (t & 63) >> 4;
@@ -335,15 +334,13 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             for (int j = 0; j < 4; ++j) Br[j] = tw<-1>(tb.nmp, (uint32_t)(1024 * (4 * kb + j) * k1));
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0) & 1023) * k1));
+                const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0, t) & 1023) * k1));
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    v[4 * g + j] = cconj(cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j])));
+                    v[4 * g + j] = cconj(cmul(lds_get(lds, lds_re(mfma_elem(g, j, t))), cmul(Ad, Br[j])));
                 }
             }
-            fft16k_mfma<+1, P>(v, lds, &ftw, W);
-            b_load(V, nbpl, nk1, nx);
-            __builtin_amdgcn_sched_barrier(0);
+            fft16k_mfma<+1, P>(v, lds, &ftw, W, t);
 #pragma unroll
             for (int q = 0; q < 16; ++q) v[q] = cconj(v[q]);
         }
@@ -714,9 +711,11 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
     }
     float pr[C2_NSLOT + 1], pi[C2_NSLOT + 1];
     int row = blockIdx.x;
-    c2_load(Bc, row, pr, pi);
+    if constexpr (P == 0) c2_load(Bc, row, pr, pi);
     for (;;) {
         const int tx = row & (KHI - 1), bpl = row / KHI;
+        // the matrix-core variants load their own row here instead of prefetching
+        if constexpr (P != 0) c2_load(Bc, row, pr, pi);
         __syncthreads();   // the previous row's LDS reads are done
 #pragma unroll
         for (int i = 0; i <= C2_NSLOT; ++i)
@@ -742,24 +741,22 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             fft16k_p4<+1>(v, &ftw, tr);
         } else {
             // MFMA layout: element c = d + 1024 r, twiddle e^{2 pi i c tx/2^20} = A(d) B(r)
-            const int l = t & 63, kb = l >> 4;
+            const int t = row_tid(), l = t & 63, kb = l >> 4;
             float2 Br[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) Br[j] = tw<+1>(tb.np, (uint32_t)(1024 * (4 * kb + j) * tx));
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int e0 = mfma_elem(g, 0);
+                const int e0 = mfma_elem(g, 0, t);
                 const float2 Ad = tw<+1>(tb.np, (uint32_t)((e0 & 1023) * tx));
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    v[4 * g + j] = cmul(lds_get(lds, lds_re(mfma_elem(g, j))), cmul(Ad, Br[j]));
+                    v[4 * g + j] = cmul(lds_get(lds, lds_re(mfma_elem(g, j, t))), cmul(Ad, Br[j]));
                 }
             }
             MfmaW<+1, P> W;
-            W.init();
-            fft16k_mfma<+1, P>(v, lds, &ftw, W);
-            c2_load(Bc, next < n_rows ? next : row, pr, pi);
-            __builtin_amdgcn_sched_barrier(0);
+            W.init(t);
+            fft16k_mfma<+1, P>(v, lds, &ftw, W, t);
         }
         if (F32) {
             GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * t;

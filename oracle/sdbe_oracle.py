@@ -964,3 +964,50 @@ def resample_trimmed(X, E, skip, scale=TRIM_SCALE):
     Xs = np.fft.rfft(t[:nseg * R].reshape(nseg, R), axis=1)
     Y = Xs[:, skip:skip + P // 2 + 1]
     return np.fft.irfft(Y, n=P, axis=1).reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# FX cross-correlation QA (software/verification/cross_corr.py,
+# verify_scan.py:231-236); Python 3 integer division where the reference
+# relies on Python 2's.
+# ---------------------------------------------------------------------------
+def corr_Xt(X0, X1, fft_window_size=32768):
+    """cross_corr.py:23-64: auto- and cross-power averaged over the windows
+    (axis 0), lag function s_0x1 = irfft(S_0x1, n) / sqrt(max s_0x0 max s_1x1)."""
+    S_0x0 = (X0 * X0.conjugate()).mean(axis=0)
+    s_0x0 = np.fft.irfft(S_0x0, n=fft_window_size).real
+    S_1x1 = (X1 * X1.conjugate()).mean(axis=0)
+    s_1x1 = np.fft.irfft(S_1x1, n=fft_window_size).real
+    S_0x1 = (X0 * X1.conjugate()).mean(axis=0)
+    s_0x1 = np.fft.irfft(S_0x1, n=fft_window_size).real / np.sqrt(s_0x0.max() * s_1x1.max())
+    return s_0x1, S_0x1
+
+
+def corr_Xt_search(X0, X1, search_range, search_avg=1, fft_window_size=32768):
+    """cross_corr.py:66-116: from the centre window of X0
+    (X0.shape[0] / 2 - search_avg / 2), correlate search_avg windows of X0
+    with the same number of X1 starting search_range[i] windows later; the
+    peak |s_0x1| of each offset."""
+    c = X0.shape[0] // 2 - search_avg // 2
+    s = np.zeros((len(search_range), fft_window_size))
+    S = np.zeros((len(search_range), X0.shape[1]), dtype=np.complex128)
+    peaks = np.zeros(len(search_range))
+    for i, iw in enumerate(search_range):
+        s[i], S[i] = corr_Xt(X0[c:c + search_avg], X1[c + iw:c + iw + search_avg], fft_window_size)
+        peaks[i] = np.abs(s[i]).max()
+    return s, S, peaks
+
+
+def corr_spectra(x, fft_window_size=32768):
+    """Positive half-spectra of the consecutive windows of x (cross_corr.py:150-151)."""
+    n = x.size // fft_window_size
+    return np.fft.fft(np.asarray(x[:n * fft_window_size], dtype=np.float64).reshape(n, fft_window_size),
+                      axis=1)[:, :fft_window_size // 2]
+
+
+def corr_snr(s, peaks):
+    """verify_scan.py:232-235: at the offset of the highest peak, the peak over
+    the standard deviation of that lag function; returns (snr, signal, index)."""
+    i = int(np.argmax(peaks))
+    signal = np.abs(s[i]).max()
+    return signal / s[i].std(), signal, i

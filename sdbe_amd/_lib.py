@@ -127,9 +127,10 @@ PFB_NTAP = 4
 REORDER2015_CHANNELS = 16384
 REORDER2015_SNAPSHOTS = 128
 REORDER2015_SPLIT = 69
+XCORR_WINDOW = 32768
 
 # (name, restype, argtypes) of every symbol declared in include/sdbe_stage.h,
-# include/sdbe_pfb.h and include/sdbe_formats.h
+# include/sdbe_pfb.h, include/sdbe_formats.h and include/sdbe_qa.h
 PROTOTYPES = [
     ("sdbe_default_config", C.c_int, [C.POINTER(Config)]),
     ("sdbe_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
@@ -161,6 +162,12 @@ PROTOTYPES = [
     ("sdbe_profile_read", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int32, C.POINTER(C.c_int32)]),
     # include/sdbe_formats.h
     ("sdbe_reorder_2015", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    # include/sdbe_qa.h
+    ("sdbe_xcorr_create", C.c_int, [C.c_int, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("sdbe_xcorr_destroy", None, [C.c_void_p]),
+    ("sdbe_xcorr_last_hip_error", C.c_int, [C.c_void_p]),
+    ("sdbe_xcorr_search", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None

@@ -45,7 +45,7 @@ def _run(cmd):
 def build_stage_lib():
     tmp = LIB + ".tmp"
     _run([_hipcc(), *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "sdbe_stage.hip"), os.path.join(CSRC, "sdbe_pfb.hip"),
-          os.path.join(CSRC, "sdbe_formats.hip")])
+          os.path.join(CSRC, "sdbe_formats.hip"), os.path.join(CSRC, "sdbe_qa.hip")])
     os.replace(tmp, LIB)
     return LIB
 

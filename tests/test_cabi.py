@@ -9,7 +9,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HDRS = [os.path.join(ROOT, "include", h) for h in ("sdbe_stage.h", "sdbe_pfb.h", "sdbe_formats.h")]
+HDRS = [os.path.join(ROOT, "include", h) for h in ("sdbe_stage.h", "sdbe_pfb.h", "sdbe_formats.h", "sdbe_qa.h")]
 
 
 def _declared_functions():

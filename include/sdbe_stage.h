@@ -299,6 +299,10 @@ typedef struct sdbe_kernel_time {
 int sdbe_profile_enable(sdbe_handle_t *h, int enable);
 int sdbe_profile_read(sdbe_handle_t *h, sdbe_kernel_time_t *out, int32_t max_records,
                       int32_t *n_records);
+/* Bracket only the launches of one kernel kind (its name as in
+ * sdbe_kernel_time_t; NULL or "" = every kernel), so that a timed run carries
+ * two events per launch of that kernel only.  SDBE_ERR_ARG for an unknown name. */
+int sdbe_profile_select(sdbe_handle_t *h, const char *kernel);
 
 #ifdef __cplusplus
 }

@@ -160,6 +160,7 @@ PROTOTYPES = [
     ("sdbe_pfb_inverse", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("sdbe_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("sdbe_profile_read", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int32, C.POINTER(C.c_int32)]),
+    ("sdbe_profile_select", C.c_int, [C.c_void_p, C.c_char_p]),
     # include/sdbe_formats.h
     ("sdbe_reorder_2015", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     # include/sdbe_qa.h

@@ -939,6 +939,7 @@ struct sdbe_handle {
     int g_used = 0;
     int c1_tpw = 8;
     int n_cu = 256;                  // compute units of the device
+    int prof_kid = -1;               // profile only this kernel kind (-1: all)
     // resampler: 64/39 (trim.D == 0) or the band-trimmed path of a rate preset
     trim::Geometry trim;
     trim::Tabs trim_tabs{};
@@ -977,7 +978,7 @@ int hip_fail(sdbe_handle_t *h, hipError_t e) {
     } while (0)
 
 ProfRec *prof_begin(sdbe_handle_t *h, int kid, double units, hipStream_t s) {
-    if (!h->prof || h->capturing) return nullptr;
+    if (!h->prof || h->capturing || (h->prof_kid >= 0 && kid != h->prof_kid)) return nullptr;
     hipEvent_t ev[2];
     for (int i = 0; i < 2; ++i) {
         if (!h->pool.empty()) {
@@ -1854,6 +1855,20 @@ int sdbe_profile_enable(sdbe_handle_t *h, int enable) {
     if (!h) return SDBE_ERR_ARG;
     h->prof = enable ? 1 : 0;
     return SDBE_OK;
+}
+
+int sdbe_profile_select(sdbe_handle_t *h, const char *kernel) {
+    if (!h) return SDBE_ERR_ARG;
+    if (!kernel || !kernel[0]) {
+        h->prof_kid = -1;
+        return SDBE_OK;
+    }
+    for (int k = 0; k < KID_N; ++k)
+        if (strcmp(kKernelNames[k], kernel) == 0) {
+            h->prof_kid = k;
+            return SDBE_OK;
+        }
+    return SDBE_ERR_ARG;
 }
 
 int sdbe_profile_read(sdbe_handle_t *h, sdbe_kernel_time_t *out, int32_t max_records, int32_t *n_records) {

@@ -104,6 +104,11 @@ class SdbeStage:
     def profile(self, enable=True):
         L.check(self.lib.sdbe_profile_enable(self.h, 1 if enable else 0), "sdbe_profile_enable", self.h)
 
+    def profile_select(self, kernel=None):
+        """Time only the launches of one kernel kind (None: every kernel)."""
+        L.check(self.lib.sdbe_profile_select(self.h, kernel.encode() if kernel else None),
+                "sdbe_profile_select", self.h)
+
     def profile_read(self):
         """{kernel name: (launches, total ms, units)} since the last read."""
         buf = (L.KernelTime * 32)()

@@ -127,20 +127,23 @@ k_pfb_c2r(const float2 *__restrict__ spec, float *__restrict__ yh, const float2 
 }
 
 // One row of the banded Cholesky factor of the Toeplitz band (a0, a1, a2, a3):
-// d0 = L(i, i), dk = L(i, i - k).  r1, r2, r3 are rows i - 1, i - 2, i - 3
-// ({1, 0, 0, 0} before the first row).
+// inv0 = 1 / L(i, i), dk = L(i, i - k).  r1, r2, r3 are rows i - 1, i - 2,
+// i - 3 ({1, 0, 0, 0} before the first row).  Keeping the reciprocal of the
+// diagonal makes a row one reciprocal square root and fused multiply-adds: the
+// rows form a dependent chain, and FP64 division and square root are
+// multi-instruction sequences that would sit on it four times per row.
 struct Row {
-    double d0, d1, d2, d3;
+    double inv0, d1, d2, d3;
 };
 
 __device__ __forceinline__ Row chol_row(int i, double a0, double a1, double a2, double a3, const Row &r1,
                                         const Row &r2, const Row &r3) {
     const double e1 = i >= 1 ? a1 : 0.0, e2 = i >= 2 ? a2 : 0.0, e3 = i >= 3 ? a3 : 0.0;
     Row r;
-    r.d3 = e3 / r3.d0;
-    r.d2 = (e2 - r.d3 * r2.d1) / r2.d0;
-    r.d1 = (e1 - r.d3 * r1.d2 - r.d2 * r1.d1) / r1.d0;
-    r.d0 = sqrt(a0 - r.d1 * r.d1 - r.d2 * r.d2 - r.d3 * r.d3);
+    r.d3 = e3 * r3.inv0;
+    r.d2 = (e2 - r.d3 * r2.d1) * r2.inv0;
+    r.d1 = (e1 - r.d3 * r1.d2 - r.d2 * r1.d1) * r1.inv0;
+    r.inv0 = rsqrt(a0 - r.d1 * r.d1 - r.d2 * r.d2 - r.d3 * r.d3);
     return r;
 }
 
@@ -155,8 +158,8 @@ k_pfb_fwd(const float *__restrict__ yh, int nblock, const double *__restrict__ a
     double u1 = 0.0, u2 = 0.0, u3 = 0.0;
     for (int s0 = 0; s0 < nblock; s0 += SEG) {
         double *ck = ckpt + (size_t)(s0 / SEG) * 12 * LB + n;
-        const double st[12] = {r1.d0, r1.d1, r1.d2, r1.d3, r2.d0, r2.d1, r2.d2, r2.d3,
-                               r3.d0, r3.d1, r3.d2, r3.d3};
+        const double st[12] = {r1.inv0, r1.d1, r1.d2, r1.d3, r2.inv0, r2.d1, r2.d2, r2.d3,
+                               r3.inv0, r3.d1, r3.d2, r3.d3};
 #pragma unroll
         for (int k = 0; k < 12; ++k) ck[(size_t)k * LB] = st[k];
         float y[SEG];
@@ -167,7 +170,7 @@ k_pfb_fwd(const float *__restrict__ yh, int nblock, const double *__restrict__ a
             const int i = s0 + k;
             if (i < nblock) {
                 const Row r = chol_row(i, a0, a1, a2, a3, r1, r2, r3);
-                const double ui = ((double)y[k] - r.d1 * u1 - r.d2 * u2 - r.d3 * u3) / r.d0;
+                const double ui = ((double)y[k] - r.d1 * u1 - r.d2 * u2 - r.d3 * u3) * r.inv0;
                 u[(size_t)i * LB + n] = ui;
                 r3 = r2;
                 r2 = r1;
@@ -212,7 +215,7 @@ k_pfb_bwd(const double *__restrict__ u, int nblock, const double *__restrict__ a
         for (int k = SEG - 1; k >= 0; --k) {
             const int i = s0 + k;
             if (i < nblock) {
-                const double zi = (uu[k] - q1.d1 * z1 - q2.d2 * z2 - q3.d3 * z3) / Lr[k].d0;
+                const double zi = (uu[k] - q1.d1 * z1 - q2.d2 * z2 - q3.d3 * z3) * Lr[k].inv0;
                 x[(size_t)(i + 3) * LB + n] = (float)(w0 * z3 + w1 * z2 + w2 * z1 + w3 * zi);
                 q3 = q2;
                 q2 = q1;

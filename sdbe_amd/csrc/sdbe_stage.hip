@@ -7,9 +7,13 @@
 //   k_reorder                   LDS transpose of 16 packets per (frame, cid pair)
 //                               into the reference int8 [snapshot][channel][im,re]
 //                               rows, missing packets -> 0 + 0j, fid bitmap
+//                               (int8 unpack entry point, band-trimmed rates)
+//   k_group_bitmap              fid bitmap and unique count only (64/39 groups,
+//                               whose stage A reads the packets)
 // then per chunk of block-pols:
 //   k_stage_a    decode + C2R pre-processing of each 32768-point snapshot
 //                transform + 39-point DFT across the block's snapshots
+//                (k_stage_a_pk: the same, decoding the group's packets in place)
 //   k_stage_b    per 39-DFT row: 16384-point inverse FFT, inter-pass twiddle,
 //                16384-point forward FFT  => packed 1,277,952-point R2C
 //   k_stage_c1   R2C post-processing, zero padding, C2R pre-processing of the
@@ -225,6 +229,112 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables
     float2 *dst = V + (size_t)bpl * NBLK * NCH + k;
 #pragma unroll
     for (int k1 = 0; k1 < NBLK; ++k1) dst[(size_t)k1 * NCH] = x[k1];
+}
+
+// Stage A of a whole group, reading the packets in place of the int8 rows
+// (k_reorder's decode fused in).  Block-pol (blk, pol) covers group snapshots
+// S0 = 39 blk .. S0 + 38, which lie in B-frames f0 = S0 / 128 and f0 + 1 (from
+// s = 128 (f0 + 1) - S0 on).  Channel c sits in packet (frame, cid, fid) with
+// g = c >> 3, cid = 2 (g >> 4) + (g & 1), fid = (g >> 1) & 7: payload words
+// 2 sn, 2 sn + 1 of snapshot sn hold its 8 channels, channel c in word
+// (c & 7) >> 2, byte 3 - (c & 3), bits from the LSB pol1 im, pol1 re, pol0 im,
+// pol0 re (vdif_inout_gpu_thread.cu:442, 464-494).  A missing packet reads as
+// 0xAA bytes, which decode to 0 + 0j (as in k_reorder).
+// The workgroup's 256 channels k lie in 32 channel groups and their mirrors
+// (or, flipped, their reversals) in 33: the 39 snapshot word pairs of those 65
+// groups are staged in LDS with coalesced loads (the packet indices first),
+// then every thread decodes its two channels from LDS.  Same workgroup mapping
+// and arithmetic as k_stage_a.
+constexpr int APK_GA = 32, APK_GB = 33, APK_G = APK_GA + APK_GB;
+
+__global__ void __launch_bounds__(256, 2)
+k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables tb,
+             const uint32_t *__restrict__ pk, const int *__restrict__ map) {
+    __shared__ uint2 seg[APK_G * NBLK];     // [group slot][snapshot] word pairs
+    __shared__ int pidx[APK_G * 2];         // [group slot][frame f0, f1] packet index
+    const int L = xcd_contig((int)blockIdx.x, (NCH / 256) * nb);
+    const int bpl = L >> 6, idx = L & 63;
+    const int w = (idx & 1) ? 63 - (idx >> 1) : (idx >> 1);
+    const int t = threadIdx.x, k = w * 256 + t;
+    const int flip = bpd[bpl].flip, pol = bpd[bpl].pol;
+    const int S0 = NBLK * bpd[bpl].blk, f0 = S0 >> 7, s_sw = 128 * (f0 + 1) - S0;
+    const int f1 = min(f0 + 1, NFR - 1);
+    // first group of each range: k (32 groups), mirror / reversed mirror (33)
+    const int ga0 = flip ? 2047 - 32 * w - 31 : 32 * w;
+    const int gb0 = flip ? 32 * w - 1 : 2048 - 32 * w - 32;
+    if (t < 2 * APK_G) {
+        const int i = t >> 1, fr = (t & 1) ? f1 : f0;
+        const int g = min(max(i < APK_GA ? ga0 + i : gb0 + (i - APK_GA), 0), NCH / 8 - 1);
+        const int cid = 2 * (g >> 4) + (g & 1), fid = (g >> 1) & 7;
+        pidx[t] = map[(fr * 256 + cid) * 8 + fid];
+    }
+    __syncthreads();
+    for (int e = t; e < APK_G * NBLK; e += 256) {
+        const int i = e / NBLK, sn = e - i * NBLK;
+        const bool first = sn < s_sw;
+        const int p = pidx[2 * i + (first ? 0 : 1)];
+        const int row = first ? S0 - 128 * f0 + sn : sn - s_sw;
+        uint2 v = make_uint2(0xAAAAAAAAu, 0xAAAAAAAAu);
+        if (p != MAP_NONE) v = *reinterpret_cast<const uint2 *>(pk + (size_t)p * PKT_WORDS + 8 + 2 * row);
+        seg[e] = v;
+    }
+    __syncthreads();
+    const int km = (NCH - k) & (NCH - 1);      // k = 0 pairs with the zero Nyquist bin
+    const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
+    // word offsets into seg (as uint32), and the nibble's bit
+    const uint32_t *seg32 = reinterpret_cast<const uint32_t *>(seg);
+    const int sa = (ka >> 3) - ga0, sb = min(max((kb >> 3) - gb0, 0), APK_GB - 1);   // k = 0: kb unused
+    const int oa = 2 * NBLK * sa + ((ka & 7) >> 2), ob = 2 * NBLK * (APK_GA + sb) + ((kb & 7) >> 2);
+    const int pbit = pol == 0 ? 4 : 0;
+    const int sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
+    const float kz = k == 0 ? 0.f : 1.f;
+    const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
+    float2 x[NBLK];
+#pragma unroll
+    for (int s = 0; s < NBLK; ++s) {
+        const uint32_t ua = seg32[oa + 2 * s] >> sha, ub = seg32[ob + 2 * s] >> shb;
+        // code -> value (code & 3) - 2: im in bits 0-1, re in bits 2-3
+        const float ia = (float)(ua & 3u) - 2.f, ra = (float)((ua >> 2) & 3u) - 2.f;
+        const float ib = (float)(ub & 3u) - 2.f, rb = (float)((ub >> 2) & 3u) - 2.f;
+        const float2 a = make_float2(ra, kz * ia);            // DC imaginary part dropped
+        const float2 b = make_float2(kz * rb, kz * ib);       // Nyquist bin is zero
+        const float2 sum = cadd(a, cconj(b));
+        const float2 dif = csub(a, cconj(b));
+        x[s] = cadd(sum, mul_si<+1>(cmul(T, dif)));
+    }
+    dft39<-1>(x);
+    float2 *dst = V + (size_t)bpl * NBLK * NCH + k;
+#pragma unroll
+    for (int k1 = 0; k1 < NBLK; ++k1) dst[(size_t)k1 * NCH] = x[k1];
+}
+
+// The fid bitmap byte of every (frame, cid) of the group and the count of
+// filled packet slots (what k_reorder records, for the packet-reading stage A).
+__global__ void __launch_bounds__(256)
+k_group_bitmap(const int *__restrict__ map, uint8_t *__restrict__ bitmap, int *__restrict__ cnt) {
+    __shared__ int red[4];
+    const int frame = blockIdx.x, cid = threadIdx.x;
+    const int4 *m = reinterpret_cast<const int4 *>(map + (frame * 256 + cid) * 8);
+    const int4 m0 = m[0], m1 = m[1];
+    const int v[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+    uint32_t byte = 0;
+    int found = 0;
+#pragma unroll
+    for (int fid = 0; fid < 8; ++fid) {
+        if (v[fid] != MAP_NONE) {
+            byte |= 1u << fid;
+            ++found;
+        }
+    }
+    bitmap[frame * 256 + cid] = (uint8_t)byte;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) found += __shfl_xor(found, o, 64);
+    if ((cid & 63) == 0) red[cid >> 6] = found;
+    __syncthreads();
+    if (cid == 0) {
+        const int tot = red[0] + red[1] + red[2] + red[3];
+        if (tot) atomicAdd(cnt + CNT_UNIQUE, tot);
+    }
 }
 
 // Word 3 of a raw buffer resource (untyped dword access, gfx9 family): loads
@@ -886,12 +996,12 @@ __global__ void __launch_bounds__(256) k_lead_copy(const uint8_t *__restrict__ l
 // ======================================================================= host
 enum KernelId {
     KID_FILL, KID_MAP, KID_REORDER, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
-    KID_PACK, KID_HEADERS, KID_TRIM_C2R, KID_TRIM_SEG, KID_TRIM_PACK, KID_N
+    KID_PACK, KID_HEADERS, KID_TRIM_C2R, KID_TRIM_SEG, KID_TRIM_PACK, KID_A_PK, KID_BITMAP, KID_N
 };
 static const char *const kKernelNames[KID_N] = {
     "k_fill_i32", "k_packet_map", "k_reorder", "k_stage_a", "k_stage_b",
     "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers", "k_trim_c2r", "k_trim_seg",
-    "k_trim_pack"};
+    "k_trim_pack", "k_stage_a_pk", "k_group_bitmap"};
 
 struct ProfRec {
     int kid;
@@ -1179,14 +1289,20 @@ int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od,
 
 // Run stages A..C2 (+ pack) over the n_bp block-pols described by the device
 // table d_bpd, in chunks.  f32: write float samples instead of codes.
+// pk: the group's packets when stage A reads them directly (the block-pols'
+// blk / pol then locate their snapshots through the packet map), else null.
 int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32,
-                  double *d_stats, hipStream_t s) {
+                  double *d_stats, hipStream_t s, const uint32_t *pk = nullptr) {
     if (h->trim.frac == 6) return run_trim<39, 32>(h, d_bpd, n_bp, od, f32, d_stats, s);
     if (h->trim.frac == 8) return run_trim<13, 8>(h, d_bpd, n_bp, od, f32, d_stats, s);
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
-        LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb);
+        if (pk)
+            LAUNCH(h, KID_A_PK, nb, s, k_stage_a_pk, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb, pk,
+                   (const int *)h->map);
+        else
+            LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb);
         {
             const int rows = NBLK * nb;
             const int grid = rows < h->n_cu ? rows : h->n_cu;
@@ -1293,15 +1409,35 @@ sdbe_stream_state_t state_plus(const sdbe_stream_state_t &st, int64_t n_frames) 
     return t;
 }
 
-int unpack_to_rows(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero,
-                   uint8_t *d_bitmap, hipStream_t s) {
+// The group's packet index map [frame][cid][fid] and counters; then either
+// the int8 rows (k_reorder) or, for the packet-reading stage A, only the fid
+// bitmap and the unique count.
+int unpack_map(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero, hipStream_t s) {
     LAUNCH(h, KID_FILL, 1, s, k_fill_i32, dim3((MAP_N + 255) / 256), dim3(256), h->map, MAP_N, MAP_NONE);
     LAUNCH(h, KID_FILL, 0, s, k_fill_i32, dim3(1), dim3(256), h->cnt, CNT_N, 0);
     if (n_pkts > 0)
         LAUNCH(h, KID_MAP, 1, s, k_packet_map, dim3((n_pkts + 255) / 256), dim3(256), (const uint32_t *)d_pkts,
                n_pkts, (uint32_t)(b_zero & 0xFFFFFFFFull), h->map, h->cnt);
+    HIP_TRY(h, hipGetLastError());
+    return SDBE_OK;
+}
+
+int unpack_to_rows(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero,
+                   uint8_t *d_bitmap, hipStream_t s) {
+    int rc = unpack_map(h, d_pkts, n_pkts, b_zero, s);
+    if (rc) return rc;
     LAUNCH(h, KID_REORDER, 1, s, k_reorder, dim3(128, NFR), dim3(256), (const uint32_t *)d_pkts,
            (const int *)h->map, h->beng, d_bitmap ? d_bitmap : h->bitmap, h->cnt);
+    HIP_TRY(h, hipGetLastError());
+    return SDBE_OK;
+}
+
+int unpack_bitmap(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero,
+                  uint8_t *d_bitmap, hipStream_t s) {
+    int rc = unpack_map(h, d_pkts, n_pkts, b_zero, s);
+    if (rc) return rc;
+    LAUNCH(h, KID_BITMAP, 1, s, k_group_bitmap, dim3(NFR), dim3(256), (const int *)h->map,
+           d_bitmap ? d_bitmap : h->bitmap, h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
 }
@@ -1576,7 +1712,10 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     if (out_pol_stride_bytes < (int64_t)blocks * h->fpb * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    int rc = unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
+    // 64/39: stage A reads the packets; band-trimmed: the int8 rows
+    const bool direct = h->trim.frac == 0 && n_pkts > 0;
+    int rc = direct ? unpack_bitmap(h, d_pkts, n_pkts, b_zero, d_bitmap, s)
+                    : unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;
     int slot;
     if ((rc = ring_acquire(h, slot))) return rc;
@@ -1621,7 +1760,8 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
         od.lead = h->lead;
         od.bp_headers = 0;
     }
-    rc = run_resampler(h, h->d_bpd[slot], n_bp, od, false, nullptr, s);
+    rc = run_resampler(h, h->d_bpd[slot], n_bp, od, false, nullptr, s,
+                       direct ? (const uint32_t *)d_pkts : nullptr);
     if (rc) return rc;
     if ((rc = ring_release(h, slot, s))) return rc;
     if (!od.bp_headers && od.hdr_bytes) {

@@ -255,6 +255,48 @@ def test_process_group_order_invariant_and_deterministic(torch_cuda, one_group):
     st.close()
 
 
+def test_process_group_packet_stage_a_matches_rows(torch_cuda, one_group):
+    """sdbe_process_group's stage A decodes the packets in place (through the
+    packet map, a missing packet as 0 + 0j); it must equal stage A over the
+    reordered int8 rows of the same group bit for bit, for all 128 blocks of both
+    polarisations (including the blocks whose 39 snapshots straddle two
+    B-frames), and give the same fid bitmap and counters."""
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    pk, pk_shuf, b0 = one_group
+    # also drop every fid of one cid in the two frames under block 3 (snapshots 117..155)
+    keep = np.ones(pk_shuf.shape[0], dtype=bool)
+    w5 = pk_shuf[:, 5]
+    bc = ((w5 >> 24) + ((pk_shuf[:, 4] & 0xFFFFFF) << 8)).astype(np.int64) - b0
+    keep &= ~(((bc == 0) | (bc == 1)) & ((w5 & 0xFF) == 77))
+    pk_cut = np.ascontiguousarray(pk_shuf[keep])
+    st = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD)
+    d = st.to_device_packets(pk_cut)
+    bm = torch.empty((39, 256), dtype=torch.uint8, device="cuda")
+    _say("[group-pk] gpu process_group, n_pkts=%d" % pk_cut.shape[0])
+    out = st.process_group(d, pk_cut.shape[0], b0, bitmap=bm).cpu().numpy()
+    stats = st.stats()
+    ref = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD)
+    _say("[group-pk] gpu unpack_int8 + process_spectra per polarisation")
+    g0, g1, bm2 = ref.unpack_int8(d, pk_cut.shape[0], b0)
+    ref_stats = ref.stats()
+    for pol, g in enumerate((g0, g1)):
+        rows = g.reshape(128, 39, 16384, 2).contiguous()
+        want = ref.process_spectra(rows).cpu().numpy().reshape(-1)
+        got = out[pol].reshape(-1)
+        nd = int(np.count_nonzero(got != want))
+        _say("[group-pk] pol %d: %d bytes differ" % (pol, nd))
+        assert nd == 0
+    assert np.array_equal(bm.cpu().numpy(), bm2.cpu().numpy())
+    oracle = O.vdif_to_beng(pk_cut, b0)
+    assert np.array_equal(bm.cpu().numpy(), oracle.bitmap)
+    for key in ("n_unique", "n_missing", "n_duplicate", "n_invalid"):
+        assert stats[key] == ref_stats[key] == getattr(oracle, key)
+    assert oracle.n_missing > round(0.001 * 79872)
+    st.close()
+    ref.close()
+
+
 def test_process_group_frames_headers(torch_cuda, one_group):
     from sdbe_amd import _lib as L
     pk, pk_shuf, b0 = one_group

@@ -700,43 +700,51 @@ k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, doub
     }
 }
 
-// Threshold policy over a chunk, in block-pol order (one thread: the REFERENCE
-// latch is sequential by definition).  REFERENCE: the first polarisation-0
-// block-pol seen while the slot's threshold is 0 latches its sigma and mean,
-// which then serve both polarisations (vdif_inout_gpu_thread.cu:954-1005).
-// The chunk's statistics and policy fields are staged in LDS by all threads
-// first, so that the sequential pass does not wait on one load per block-pol.
+// Threshold policy over a chunk.  PER_BLOCK and FIXED are per block-pol and
+// run in parallel.  REFERENCE is sequential in block-pol order: the first
+// polarisation-0 block-pol seen while its slot's threshold is 0 latches its
+// sigma and mean, which then serve both polarisations
+// (vdif_inout_gpu_thread.cu:954-1005).  One thread walks the chunk for it,
+// over the statistics, policy fields and latch slots staged in LDS, so that
+// no step of the walk waits on global memory; the slots go back at the end.
 constexpr int THRESH_MAX_CHUNK = 256;
 __global__ void __launch_bounds__(256) k_thresh(const double2 *__restrict__ ms, int nbpl,
                                                 const BpDesc *__restrict__ bpd, float *__restrict__ latch,
                                                 float2 *__restrict__ thr) {
     __shared__ float2 sms[THRESH_MAX_CHUNK];      // (sigma, mean) as float
-    __shared__ float2 sfix[THRESH_MAX_CHUNK];
-    __shared__ int sinfo[THRESH_MAX_CHUNK];       // policy | pol << 8 | slot << 16
-    for (int b = threadIdx.x; b < nbpl; b += blockDim.x) {
+    __shared__ int sinfo[THRESH_MAX_CHUNK];       // pol | slot << 8, or -1: not REFERENCE
+    __shared__ float2 slat[SDBE_LATCH_SLOTS];
+    __shared__ int any_ref;
+    const int t = threadIdx.x;
+    if (t == 0) any_ref = 0;
+    if (t < SDBE_LATCH_SLOTS) slat[t] = make_float2(latch[2 * t], latch[2 * t + 1]);
+    __syncthreads();
+    for (int b = t; b < nbpl; b += blockDim.x) {
         const BpDesc &d = bpd[b];
-        sms[b] = make_float2((float)ms[b].y, (float)ms[b].x);
-        sfix[b] = make_float2(d.fixed_th, d.fixed_off);
-        sinfo[b] = (int)d.policy | ((int)d.pol << 8) | ((int)d.slot << 16);
+        const float2 v = make_float2((float)ms[b].y, (float)ms[b].x);
+        sms[b] = v;
+        if (d.policy == SDBE_THRESH_REFERENCE) {
+            sinfo[b] = (int)d.pol | ((int)d.slot << 8);
+            any_ref = 1;
+        } else {
+            sinfo[b] = -1;
+            thr[b] = d.policy == SDBE_THRESH_PER_BLOCK ? v : make_float2(d.fixed_th, d.fixed_off);
+        }
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (t != 0 || !any_ref) return;
     for (int b = 0; b < nbpl; ++b) {
-        const int info = sinfo[b], policy = info & 0xFF, pol = (info >> 8) & 0xFF, slot = info >> 16;
-        float2 to;
-        if (policy == SDBE_THRESH_PER_BLOCK) {
-            to = sms[b];
-        } else if (policy == SDBE_THRESH_REFERENCE) {
-            float *l = latch + 2 * slot;
-            if (l[0] == 0.0f && pol == 0) {
-                l[0] = sms[b].x;
-                l[1] = sms[b].y;
-            }
-            to = make_float2(l[0], l[1]);
-        } else {
-            to = sfix[b];
+        const int info = sinfo[b];
+        if (info < 0) continue;
+        const int pol = info & 0xFF, slot = info >> 8;
+        float2 l = slat[slot];
+        if (l.x == 0.0f && pol == 0) {
+            l = sms[b];
+            slat[slot] = l;
+            latch[2 * slot] = l.x;
+            latch[2 * slot + 1] = l.y;
         }
-        thr[b] = to;
+        thr[b] = l;
     }
 }
 
@@ -892,39 +900,51 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
     }
 }
 
-// Pack: workgroup = (64 consecutive ty, block-pol).  Stream byte q = t/2 of a
-// block holds samples 4q..4q+3 (sample j at bits 2j, the uint32 packing of
-// vdif_inout_gpu_thread.cu:578-580): byte (i + 32 ty) = code(t = 2i + 64 ty) |
-// code(t = 2i + 1 + 64 ty) << 4.  Block bytes go to 64 frames of 8192 bytes.
+// Pack: the stage holds, per block-pol, 64 rows r of 16384 bytes S[r][ty];
+// byte S[r][ty] holds the 2-bit codes of samples 2p, 2p + 1 (p = r + 64 ty) in
+// its low nibble.  Stream byte q of a block holds samples 4q..4q+3 (sample j at
+// bits 2j, the uint32 packing of vdif_inout_gpu_thread.cu:578-580):
+// byte (i + 32 ty) = S[2i][ty] | S[2i+1][ty] << 4.
+// Thread = (block-pol, columns ty = 4c .. 4c+3): 64 row words in (a wave reads
+// 256 contiguous bytes of a row per load), M_i = S[2i] | S[2i+1] << 4 merges
+// the nibbles bytewise (codes are < 16), and byte j of M_i is stream byte
+// i + 32 (4c + j): the thread's 128 contiguous stream bytes are a 4 x 4 byte
+// transpose of M per output word.  Block bytes go to 64 frames of 8192 bytes;
+// re-framed output (shift, lead) takes them word by word.
+__device__ __forceinline__ uint32_t pack_word(const uint32_t (&M)[32], int w) {
+    const int j = w >> 3, i0 = (w & 7) * 4;
+    return ((M[i0] >> (8 * j)) & 0xFFu) | (((M[i0 + 1] >> (8 * j)) & 0xFFu) << 8) |
+           (((M[i0 + 2] >> (8 * j)) & 0xFFu) << 16) | (((M[i0 + 3] >> (8 * j)) & 0xFFu) << 24);
+}
+
 __global__ void __launch_bounds__(256)
 k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb, OutDesc od) {
-    __shared__ uint8_t tile[64][65];
-    // neighbouring ty ranges share 128-byte lines of the code rows: keep them
-    // on one XCD, back to back
-    const int L = xcd_contig((int)blockIdx.x, (NCH / 64) * nb);
-    const int bpl = L >> 8, ty0 = (L & 255) * 64, t = threadIdx.x;
-    const uint8_t *src = stage + (size_t)bpl * KHI * NCH + ty0;
+    const int bpl = blockIdx.x >> 4, c = ((blockIdx.x & 15) << 8) + threadIdx.x;   // 4096 column quads
+    const GAS uint32_t *src = gptr(reinterpret_cast<const uint32_t *>(stage + (size_t)bpl * KHI * NCH)) + c;
+    uint32_t M[32];
 #pragma unroll
-    for (int it = 0; it < 16; ++it) {
-        const int e = it * 256 + t, tx = e >> 6, ty = e & 63;
-        tile[ty][tx] = src[(size_t)tx * NCH + ty];
-    }
-    __syncthreads();
-    // 64 ty x 32 bytes = 2048 contiguous stream bytes starting at 32 * ty0
+    for (int i = 0; i < 32; ++i) M[i] = src[(size_t)(2 * i) * (NCH / 4)] | (src[(size_t)(2 * i + 1) * (NCH / 4)] << 4);
     GAS uint8_t *ob = gptr(bpd[bpl].out) + od.hdr_bytes;
+    const int64_t qb0 = 128 * (int64_t)c;                 // stream byte of the run, relative to the block
+    if (od.shift_bytes == 0 && od.lim_bytes == INT64_MAX && (((uintptr_t)ob) & 15) == 0) {
+        GAS u32x4 *dst = reinterpret_cast<GAS u32x4 *>(ob + (qb0 >> 13) * od.frame_bytes + (qb0 & (OUT_FRAME_PAYLOAD - 1)));
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            u32x4 q;
+            q.x = pack_word(M, 4 * v);
+            q.y = pack_word(M, 4 * v + 1);
+            q.z = pack_word(M, 4 * v + 2);
+            q.w = pack_word(M, 4 * v + 3);
+            dst[v] = q;
+        }
+        return;
+    }
     const int64_t blk_pos = (int64_t)bpd[bpl].blk * (OUT_FRAMES_BLOCK * OUT_FRAME_PAYLOAD);
     const int pol = bpd[bpl].pol;
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const int e = it * 256 + t;            // 4-byte word index within the 2048 bytes
-        const int ty = e >> 3, i0 = (e & 7) * 4;
-        uint32_t wv = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int i = i0 + b;
-            wv |= (uint32_t)(tile[ty][2 * i] | (tile[ty][2 * i + 1] << 4)) << (8 * b);
-        }
-        const int64_t qb = 32 * (ty0 + ty) + i0 + od.shift_bytes;   // stream byte, relative to the block
+    for (int w = 0; w < 32; ++w) {
+        const uint32_t wv = pack_word(M, w);
+        const int64_t qb = qb0 + 4 * w + od.shift_bytes;           // stream byte, relative to the block
         const int64_t gpos = blk_pos + qb;                          // ... and to the group
         if (gpos < 0) continue;
         if (gpos >= od.lim_bytes) {
@@ -1334,8 +1354,7 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
             launch_c2<true>(h, nb, s, bpd);
         } else {
             launch_c2<false>(h, nb, s, bpd);
-            LAUNCH(h, KID_PACK, nb, s, k_pack, dim3((NCH / 64) * nb), dim3(256), (const uint8_t *)h->stage, bpd, nb,
-                   od);
+            LAUNCH(h, KID_PACK, nb, s, k_pack, dim3(16 * nb), dim3(256), (const uint8_t *)h->stage, bpd, nb, od);
         }
         HIP_TRY(h, hipGetLastError());
     }

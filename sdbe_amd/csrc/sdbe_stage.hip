@@ -927,15 +927,25 @@ k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb
     GAS uint8_t *ob = gptr(bpd[bpl].out) + od.hdr_bytes;
     const int64_t qb0 = 128 * (int64_t)c;                 // stream byte of the run, relative to the block
     if (od.shift_bytes == 0 && od.lim_bytes == INT64_MAX && (((uintptr_t)ob) & 15) == 0) {
-        GAS u32x4 *dst = reinterpret_cast<GAS u32x4 *>(ob + (qb0 >> 13) * od.frame_bytes + (qb0 & (OUT_FRAME_PAYLOAD - 1)));
+        // the workgroup's 32 KB of stream bytes through LDS, so that a wave
+        // stores 1 KB contiguous per instruction (row stride 33 words)
+        __shared__ uint32_t runs[256 * 33];
+        const int t = threadIdx.x;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
+        for (int w = 0; w < 32; ++w) runs[t * 33 + w] = pack_word(M, w);
+        __syncthreads();
+        const int64_t wg0 = 128 * (int64_t)((blockIdx.x & 15) << 8);     // first stream byte of the workgroup
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int e = it * 1024 + 4 * t;                                // word of the workgroup's run
+            const int src = (e >> 5) * 33 + (e & 31);
             u32x4 q;
-            q.x = pack_word(M, 4 * v);
-            q.y = pack_word(M, 4 * v + 1);
-            q.z = pack_word(M, 4 * v + 2);
-            q.w = pack_word(M, 4 * v + 3);
-            dst[v] = q;
+            q.x = runs[src];
+            q.y = runs[src + 1];
+            q.z = runs[src + 2];
+            q.w = runs[src + 3];
+            const int64_t qb = wg0 + 4 * e;
+            *reinterpret_cast<GAS u32x4 *>(ob + (qb >> 13) * od.frame_bytes + (qb & (OUT_FRAME_PAYLOAD - 1))) = q;
         }
         return;
     }

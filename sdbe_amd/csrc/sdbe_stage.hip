@@ -192,6 +192,35 @@ __device__ __forceinline__ GAS T *gptr(T *p) {
     return (GAS T *)p;
 }
 
+// Word 3 of a raw buffer resource (untyped dword access, gfx9 family): loads
+// through it are range-checked against the resource's size and read 0 past it.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+
+// Stage A's per-snapshot input x_s = sum_s + i T dif_s, with
+//   a = (Ra, kz Ia), b = (kz Rb, kz Ib), sum = a + conj b, dif = a - conj b.
+// sum and dif are small integers, exact in FP32 in any order of operations: the
+// packet path passes 2-bit codes (value + 2) and folds the -2 offsets into the
+// per-lane constants cs, cd; the int8 path passes values and zero constants.
+// Both give exactly the same float inputs to the first rounding, cmul(T, dif).
+__device__ __forceinline__ float2 a_input(float Ra, float Ia, float Rb, float Ib, float kz, float2 cs, float2 cd,
+                                          float2 T) {
+    const float2 sum = make_float2(fmaf(kz, Rb, Ra) + cs.x, kz * (Ia - Ib));          // DC imaginary dropped,
+    const float2 dif = make_float2(fmaf(-kz, Rb, Ra) + cd.x, fmaf(kz, Ia + Ib, cd.y));  // Nyquist bin zero
+    return cadd(sum, mul_si<+1>(cmul(T, dif)));
+}
+
+typedef unsigned int u32v2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+
+// V[bpl][k1][k] for k1 = 0..38: buffer stores, the row in the scalar offset.
+__device__ __forceinline__ void a_store(float2 *__restrict__ V, int bpl, int k, const float2 (&x)[NBLK]) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(V + (size_t)bpl * NBLK * NCH), 0, NBLK * NCH * (int)sizeof(float2), BUF_RSRC_WORD3);
+#pragma unroll
+    for (int k1 = 0; k1 < NBLK; ++k1)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32v2, x[k1]), rs, 8 * k,
+                                              k1 * NCH * (int)sizeof(float2), 0);
+}
+
 // Stage A: thread per channel k of a block-pol.  For each snapshot s the
 // packed C2R input Z_s[k] = (X[k] + conj(X[N2-k])) + i e^{2 pi i k/32768}
 // (X[k] - conj(X[N2-k])) (DC imaginary part and the zero Nyquist bin dropped,
@@ -201,7 +230,7 @@ __device__ __forceinline__ GAS T *gptr(T *p) {
 // Workgroups are paired (w, 63 - w) on one XCD back to back, so the mirror
 // channels N2 - k a workgroup reads are the ones its partner reads as k and
 // are served from that XCD's L2.
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, 4)
 k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables tb) {
     const int L = xcd_contig((int)blockIdx.x, (NCH / 256) * nb);
     const int bpl = L >> 6, idx = L & 63;
@@ -217,18 +246,12 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables
 #pragma unroll
     for (int s = 0; s < NBLK; ++s) {
         const uint16_t ua = src[(size_t)s * NCH + ka], ub = src[(size_t)s * NCH + kb];
-        const char2 va = make_char2((int8_t)(ua & 0xFF), (int8_t)(ua >> 8));   // (im, re)
-        const char2 vb = make_char2((int8_t)(ub & 0xFF), (int8_t)(ub >> 8));
-        const float2 a = make_float2((float)va.y, kz * (float)va.x);     // DC imaginary part dropped
-        const float2 b = make_float2(kz * (float)vb.y, kz * (float)vb.x);  // Nyquist bin is zero
-        const float2 sum = cadd(a, cconj(b));
-        const float2 dif = csub(a, cconj(b));
-        x[s] = cadd(sum, mul_si<+1>(cmul(T, dif)));
+        const float Ia = (float)(int8_t)(ua & 0xFF), Ra = (float)(int8_t)(ua >> 8);   // (im, re) values
+        const float Ib = (float)(int8_t)(ub & 0xFF), Rb = (float)(int8_t)(ub >> 8);
+        x[s] = a_input(Ra, Ia, Rb, Ib, kz, make_float2(0.f, 0.f), make_float2(0.f, 0.f), T);
     }
     dft39<-1>(x);
-    float2 *dst = V + (size_t)bpl * NBLK * NCH + k;
-#pragma unroll
-    for (int k1 = 0; k1 < NBLK; ++k1) dst[(size_t)k1 * NCH] = x[k1];
+    a_store(V, bpl, k, x);
 }
 
 // Stage A of a whole group, reading the packets in place of the int8 rows
@@ -289,23 +312,21 @@ k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tab
     const int sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
     const float kz = k == 0 ? 0.f : 1.f;
     const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
+    // the codes' -2 offsets in sum and dif
+    const float2 cs = make_float2(-2.f - 2.f * kz, 0.f), cd = make_float2(-2.f + 2.f * kz, -4.f * kz);
     float2 x[NBLK];
 #pragma unroll
     for (int s = 0; s < NBLK; ++s) {
-        const uint32_t ua = seg32[oa + 2 * s] >> sha, ub = seg32[ob + 2 * s] >> shb;
-        // code -> value (code & 3) - 2: im in bits 0-1, re in bits 2-3
-        const float ia = (float)(ua & 3u) - 2.f, ra = (float)((ua >> 2) & 3u) - 2.f;
-        const float ib = (float)(ub & 3u) - 2.f, rb = (float)((ub >> 2) & 3u) - 2.f;
-        const float2 a = make_float2(ra, kz * ia);            // DC imaginary part dropped
-        const float2 b = make_float2(kz * rb, kz * ib);       // Nyquist bin is zero
-        const float2 sum = cadd(a, cconj(b));
-        const float2 dif = csub(a, cconj(b));
-        x[s] = cadd(sum, mul_si<+1>(cmul(T, dif)));
+        const uint32_t wa = seg32[oa + 2 * s], wb = seg32[ob + 2 * s];
+        // codes: im in the nibble's bits 0-1, re in bits 2-3
+        const float Ia = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha, 2u);
+        const float Ra = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha + 2u, 2u);
+        const float Ib = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb, 2u);
+        const float Rb = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb + 2u, 2u);
+        x[s] = a_input(Ra, Ia, Rb, Ib, kz, cs, cd, T);
     }
     dft39<-1>(x);
-    float2 *dst = V + (size_t)bpl * NBLK * NCH + k;
-#pragma unroll
-    for (int k1 = 0; k1 < NBLK; ++k1) dst[(size_t)k1 * NCH] = x[k1];
+    a_store(V, bpl, k, x);
 }
 
 // The fid bitmap byte of every (frame, cid) of the group and the count of
@@ -337,9 +358,6 @@ k_group_bitmap(const int *__restrict__ map, uint8_t *__restrict__ bitmap, int *_
     }
 }
 
-// Word 3 of a raw buffer resource (untyped dword access, gfx9 family): loads
-// through it are range-checked against the resource's size and read 0 past it.
-constexpr int BUF_RSRC_WORD3 = 0x00020000;
 
 // Stage B: row k1 of block-pol bpl: w = IFFT16384(V[k1]); w[n] *= e^{-2 pi i n k1/638976};
 // Zf[k1 + 39 k2] = FFT16384(w)[k2]   (packed forward DFT of the 1,277,952 samples)

@@ -195,6 +195,9 @@ __device__ __forceinline__ GAS T *gptr(T *p) {
 // Word 3 of a raw buffer resource (untyped dword access, gfx9 family): loads
 // through it are range-checked against the resource's size and read 0 past it.
 constexpr int BUF_RSRC_WORD3 = 0x00020000;
+// Cache-policy operand of the buffer intrinsics that selects nt (non-temporal)
+// on gfx950.
+constexpr int CPOL_NT = 2;
 
 // Stage A's per-snapshot input x_s = sum_s + i T dif_s, with
 //   a = (Ra, kz Ia), b = (kz Rb, kz Ib), sum = a + conj b, dif = a - conj b.
@@ -211,14 +214,16 @@ __device__ __forceinline__ float2 a_input(float Ra, float Ia, float Rb, float Ib
 
 typedef unsigned int u32v2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
 
-// V[bpl][k1][k] for k1 = 0..38: buffer stores, the row in the scalar offset.
+// V[bpl][k1][k] for k1 = 0..38: buffer stores, the row in the scalar offset,
+// non-temporal so that the streamed output does not evict the input lines
+// that the paired workgroup reads again.
 __device__ __forceinline__ void a_store(float2 *__restrict__ V, int bpl, int k, const float2 (&x)[NBLK]) {
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(V + (size_t)bpl * NBLK * NCH), 0, NBLK * NCH * (int)sizeof(float2), BUF_RSRC_WORD3);
 #pragma unroll
     for (int k1 = 0; k1 < NBLK; ++k1)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32v2, x[k1]), rs, 8 * k,
-                                              k1 * NCH * (int)sizeof(float2), 0);
+                                              k1 * NCH * (int)sizeof(float2), CPOL_NT);
 }
 
 // Stage A: thread per channel k of a block-pol.  For each snapshot s the
@@ -646,6 +651,8 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
         // write the tile's 32 slots of every row tx: Bc rows are planar (real
         // plane then imaginary plane, BC_ROW floats each): two 128-byte runs.
         // Buffer stores: one VGPR offset, row tx = 8 it + rw in the scalar offset.
+        // They are non-temporal (cache policy nt): the output streams out and
+        // should not evict the packed-spectrum lines the next tile reads again.
         {
             const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(Bc + (size_t)x.bpl * KHI * 2 * BC_ROW), 0, KHI * 2 * BC_ROW * (int)sizeof(float),
@@ -655,9 +662,10 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
 #pragma unroll
                 for (int it = 0; it < 8; ++it) {
                     const float2 y = sc[col][it * 8 + rw];
-                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(y.x), rb, vo, 4 * (it * 8 * 2 * BC_ROW), 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(y.x), rb, vo, 4 * (it * 8 * 2 * BC_ROW),
+                                                          CPOL_NT);
                     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(y.y), rb, vo,
-                                                          4 * (it * 8 * 2 * BC_ROW + BC_ROW), 0);
+                                                          4 * (it * 8 * 2 * BC_ROW + BC_ROW), CPOL_NT);
                 }
             }
         }

@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--c1-tiles", type=int, default=0,
+                    help="tuning: stage-C1 tiles per workgroup (0: the library default)")
     ap.add_argument("--policy", default="per_block", choices=["per_block", "reference"])
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1, 2, 3],
                     help="FFT arithmetic: 0 FP32 (default), 1 BF16X3, 2 BF16, 3 FP16 matrix-core passes")
@@ -164,7 +166,8 @@ def main():
     # one handle (own buffers, own VDIF time state) per stream; group g of a
     # step runs on stream g mod streams
     sts = [SdbeStage(policy=policy, out_mode=L.OUT_FRAMES, chunk=args.chunk, station="Sm",
-                     fft_precision=args.precision, rate_frac=args.rate, skip_mhz=args.skip_mhz)
+                     fft_precision=args.precision, rate_frac=args.rate, skip_mhz=args.skip_mhz,
+                     c1_tiles_per_wg=args.c1_tiles)
            for _ in range(args.streams)]
     st = sts[0]
     streams = [torch.cuda.Stream() for _ in range(args.streams)]
@@ -272,6 +275,7 @@ def main():
             "packets_per_step": int(n_pkts),
             "threshold_policy": args.policy,
             "chunk_blockpols": args.chunk,
+            "c1_tiles_per_wg": args.c1_tiles or 8,
             "streams": args.streams,
             "kernel_table": "warmup steps, every launch timed; roofline kernel timed in the timed steps",
             "fft_precision": ["fp32", "bf16x3", "bf16", "fp16"][args.precision],

@@ -240,6 +240,14 @@ template <int S> __device__ __forceinline__ void dft39(float2 *x) {
     for (int k = 0; k < 39; ++k) x[k] = t[k];
 }
 
+// Workgroup `lin` of a 1D grid of `total` runs on XCD lin % 8 (round-robin
+// dispatch).  The returned logical index is contiguous per XCD: logical
+// neighbours share that XCD's L2 and are dispatched close together in time.
+__device__ __forceinline__ int xcd_contig(int lin, int total) {
+    const int x = lin & 7, slot = lin >> 3, q = total >> 3, r = total & 7;
+    return x * q + (x < r ? x : r) + slot;
+}
+
 // ------------------------------------------- LDS twiddles of the 16384-point FFT
 // exp(+2 pi i m / N) for the three twiddled passes, computed in FP64 on the
 // host (640 entries, this order) and copied to LDS once per workgroup:

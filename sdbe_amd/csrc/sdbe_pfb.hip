@@ -59,13 +59,16 @@ static_assert(NH == NCH, "the PFB reuses the 16384-point FFT");
 // Forward PFB of spectrum i.  z[m] = y[2m] + i y[2m+1] of the folded block;
 // Z = FFT_16384(z); X[k] = 1/2 (Z[k] + conj Z[N-k]) - 1/2 i e^{-2 pi i k/L} (Z[k] - conj Z[N-k]),
 // X[N] = Re Z[0] - Im Z[0].
+// Spectra are contiguous per XCD (xcd_contig): the workgroups in flight on an
+// XCD fold neighbouring spectra, whose 4-block input windows overlap by 3
+// blocks, so most block reads are hits in that XCD's L2.
 __global__ void __launch_bounds__(1024)
 k_pfb_fold_r2c(const float *__restrict__ x, const float *__restrict__ win, float2 *__restrict__ spec,
                const float2 *__restrict__ fft_img, const float2 *__restrict__ tw32k) {
     __shared__ float lds[LDS_FFT_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, fft_img);
-    const int i = blockIdx.x, t = threadIdx.x;
+    const int i = xcd_contig((int)blockIdx.x, (int)gridDim.x), t = threadIdx.x;
     const float2 *xb = reinterpret_cast<const float2 *>(x + (size_t)i * LB);
     const float2 *wb = reinterpret_cast<const float2 *>(win);
     float2 v[16];

@@ -156,13 +156,6 @@ k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, int8_t *
 }
 
 // ---------------------------------------------------------------- resampler
-// Workgroup `lin` of a 1D grid of `total` runs on XCD lin % 8 (round-robin
-// dispatch).  The returned logical index is contiguous per XCD: logical
-// neighbours share that XCD's L2 and are dispatched close together in time.
-__device__ __forceinline__ int xcd_contig(int lin, int total) {
-    const int x = lin & 7, slot = lin >> 3, q = total >> 3, r = total & 7;
-    return x * q + (x < r ? x : r) + slot;
-}
 
 // One block-pol of a call: where its spectra come from, where its frames go,
 // how it is thresholded and how its frames are stamped.  Built on the host for

@@ -1760,8 +1760,9 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     if (out_pol_stride_bytes < (int64_t)blocks * h->fpb * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    // 64/39: stage A reads the packets; band-trimmed: the int8 rows
-    const bool direct = h->trim.frac == 0 && n_pkts > 0;
+    // 64/39: stage A reads the packets (8-byte words: the buffer must be 8-byte
+    // aligned, as device allocations are); band-trimmed or otherwise: the int8 rows
+    const bool direct = h->trim.frac == 0 && n_pkts > 0 && (((uintptr_t)d_pkts) & 7) == 0;
     int rc = direct ? unpack_bitmap(h, d_pkts, n_pkts, b_zero, d_bitmap, s)
                     : unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;

@@ -297,6 +297,25 @@ def test_process_group_packet_stage_a_matches_rows(torch_cuda, one_group):
     ref.close()
 
 
+def test_process_group_unaligned_packets(torch_cuda, one_group):
+    """A packet buffer that is only 4-byte aligned takes the int8-row path
+    (stage A's packet reads are 8-byte words); the output is the same."""
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    _, pk_shuf, b0 = one_group
+    st = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD)
+    d = st.to_device_packets(pk_shuf)
+    raw = torch.empty(d.numel() + 4, dtype=torch.uint8, device="cuda")
+    raw[4:].copy_(d)
+    du = raw[4:]
+    assert du.data_ptr() % 8 == 4
+    _say("[group-align] gpu process_group aligned / offset by 4 bytes")
+    want = st.process_group(d, pk_shuf.shape[0], b0).cpu().numpy()
+    got = st.process_group(du, pk_shuf.shape[0], b0).cpu().numpy()
+    assert np.array_equal(got, want)
+    st.close()
+
+
 def test_process_group_frames_headers(torch_cuda, one_group):
     from sdbe_amd import _lib as L
     pk, pk_shuf, b0 = one_group

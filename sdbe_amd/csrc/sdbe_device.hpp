@@ -135,22 +135,58 @@ template <int S> __device__ __forceinline__ void dft8(float2 *v) {
 }
 
 // DFT-16: n = 4*n1 + n2, k = k1 + 4*k2
+// DFT16 as 4 x 4: radix-4 over n2, internal twiddles W16^(n2 k1), radix-4 over
+// k1.  The twiddles are fused into the second radix-4 layer: with
+// W = e^{i S pi/8}, c = cos(pi/8), tau = tan(pi/8), h = sqrt(1/2):
+//   W^1 = c (1 + i S tau), W^3 = i S c (1 - i S tau), W^9 = -c (1 + i S tau),
+//   W^2 = h (1 + i S), W^6 = h (-1 + i S), W^4 = i S,
+// so each twiddled input is (1 +- i S tau) or (+-1 + i S) times the input (two
+// FMAs or two adds), and the common factor c or h multiplies the sums of the
+// butterfly inside FMAs: 144 operations in place of 160.
 template <int S> __device__ __forceinline__ void dft16(float2 *v) {
-    const float c1 = 9.238795325e-01f, s1 = 3.826834324e-01f, h = 7.071067812e-01f;
+    constexpr float sf = (float)S;
+    const float c1 = 9.238795325e-01f, h = 7.071067812e-01f, tau = 4.142135624e-01f;
 #pragma unroll
     for (int n2 = 0; n2 < 4; ++n2) dft4<S>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
-    // twiddle W16^(n2*k1) at position 4*k1 + n2
-    v[5] = cmulc<S>(v[5], c1, s1);      // 1
-    v[6] = cmulc<S>(v[6], h, h);        // 2
-    v[7] = cmulc<S>(v[7], s1, c1);      // 3
-    v[9] = cmulc<S>(v[9], h, h);        // 2
-    v[10] = mul_si<S>(v[10]);           // 4
-    v[11] = cmulc<S>(v[11], -h, h);     // 6
-    v[13] = cmulc<S>(v[13], s1, c1);    // 3
-    v[14] = cmulc<S>(v[14], -h, h);     // 6
-    v[15] = cmulc<S>(v[15], -c1, -s1);  // 9
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) dft4<S>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    dft4<S>(v[0], v[1], v[2], v[3]);
+    // the second layer's radix-4 on (a, b, c, d) -> (t0 + t2, t1 + t3, t0 - t2, t1 - t3)
+    // with t0, t1 = a +- c and t2, t3 = (b + d), i S (b - d) given as factor * (U, W)
+    auto out = [](float2 &x0, float2 &x1, float2 &x2, float2 &x3, float2 t0, float2 t1, float f, float2 U,
+                  float2 W) {
+        x0 = make_float2(fmaf(f, U.x, t0.x), fmaf(f, U.y, t0.y));
+        x2 = make_float2(fmaf(-f, U.x, t0.x), fmaf(-f, U.y, t0.y));
+        x1 = make_float2(fmaf(f, W.x, t1.x), fmaf(f, W.y, t1.y));
+        x3 = make_float2(fmaf(-f, W.x, t1.x), fmaf(-f, W.y, t1.y));
+    };
+    {   // k1 = 1: v4, W v5, W^2 v6, W^3 v7
+        const float2 a = v[4], x5 = v[5], x6 = v[6], x7 = v[7];
+        const float2 A = make_float2(fmaf(-sf * tau, x5.y, x5.x), fmaf(sf * tau, x5.x, x5.y));   // (1 + iS tau) v5
+        const float2 B = make_float2(fmaf(sf * tau, x7.y, x7.x), fmaf(-sf * tau, x7.x, x7.y));   // (1 - iS tau) v7
+        const float2 C = make_float2(x6.x - sf * x6.y, x6.y + sf * x6.x);                       // (1 + iS) v6
+        const float2 t0 = make_float2(fmaf(h, C.x, a.x), fmaf(h, C.y, a.y));
+        const float2 t1 = make_float2(fmaf(-h, C.x, a.x), fmaf(-h, C.y, a.y));
+        const float2 U = make_float2(A.x - sf * B.y, A.y + sf * B.x);     // A + iS B
+        const float2 Vd = make_float2(A.x + sf * B.y, A.y - sf * B.x);    // A - iS B
+        out(v[4], v[5], v[6], v[7], t0, t1, c1, U, make_float2(-sf * Vd.y, sf * Vd.x));
+    }
+    {   // k1 = 2: v8, W^2 v9, W^4 v10, W^6 v11
+        const float2 a = v[8], c = make_float2(-sf * v[10].y, sf * v[10].x);
+        const float2 P = csub(v[9], v[11]), Q = cadd(v[9], v[11]);
+        const float2 R2 = make_float2(P.x - sf * Q.y, P.y + sf * Q.x);    // P + iS Q
+        const float2 R3 = make_float2(Q.x - sf * P.y, Q.y + sf * P.x);    // Q + iS P
+        out(v[8], v[9], v[10], v[11], cadd(a, c), csub(a, c), h, R2, make_float2(-sf * R3.y, sf * R3.x));
+    }
+    {   // k1 = 3: v12, W^3 v13, W^6 v14, W^9 v15
+        const float2 a = v[12], x13 = v[13], x14 = v[14], x15 = v[15];
+        const float2 B = make_float2(fmaf(sf * tau, x13.y, x13.x), fmaf(-sf * tau, x13.x, x13.y));  // (1 - iS tau) v13
+        const float2 A = make_float2(fmaf(-sf * tau, x15.y, x15.x), fmaf(sf * tau, x15.x, x15.y));  // (1 + iS tau) v15
+        const float2 C = make_float2(-x14.x - sf * x14.y, sf * x14.x - x14.y);                      // (-1 + iS) v14
+        const float2 t0 = make_float2(fmaf(h, C.x, a.x), fmaf(h, C.y, a.y));
+        const float2 t1 = make_float2(fmaf(-h, C.x, a.x), fmaf(-h, C.y, a.y));
+        const float2 U = make_float2(-sf * B.y - A.x, sf * B.x - A.y);    // iS B - A
+        const float2 Vd = make_float2(A.x - sf * B.y, A.y + sf * B.x);    // iS B + A
+        out(v[12], v[13], v[14], v[15], t0, t1, c1, U, make_float2(-sf * Vd.y, sf * Vd.x));
+    }
     float2 t[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) t[k] = v[4 * (k & 3) + (k >> 2)];

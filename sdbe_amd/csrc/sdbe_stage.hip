@@ -571,6 +571,11 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
     }
     C1Tile cur;
     float2 z[5], cm, cl;
+    // Parseval sums of the tiles run so far for the current block-pol: reduced
+    // across the workgroup once per block-pol (a workgroup's tiles span at most
+    // two), into the slot of that block-pol's last tile here; its other slots
+    // get zeros, so the per-tile slots still sum to the block-pol's total
+    double acc_run = 0.0, dc_run = 0.0;
     if (first < last) c1_fetch(Zf, tb, first, col, rw, cur, z, cm, cl);
     for (int lt = first; lt < last; ++lt) {
         const C1Tile x = cur;
@@ -662,21 +667,32 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
                 }
             }
         }
-        // Parseval partial: wave reduce then cross-wave
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            acc += __shfl_xor(acc, off);
-            dc += __shfl_xor(dc, off);
-        }
-        if ((t & 63) == 0) {
-            red[0][t >> 6] = acc;
-            red[1][t >> 6] = dc;
-        }
+        acc_run += acc;
+        dc_run += dc;
+        // the next tile writes sz, which is sc's storage: all waves are done with sc
         __syncthreads();
-        if (t == 0) {
-            double *o = part + ((size_t)x.bpl * C1_TILES + x.tile) * 2;
-            o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-            o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        double *o = part + ((size_t)x.bpl * C1_TILES + x.tile) * 2;
+        if (lt + 1 == last || (lt + 1) / C1_TILES != x.bpl) {   // workgroup-uniform
+            // wave reduce then cross-wave
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                acc_run += __shfl_xor(acc_run, off);
+                dc_run += __shfl_xor(dc_run, off);
+            }
+            if ((t & 63) == 0) {
+                red[0][t >> 6] = acc_run;
+                red[1][t >> 6] = dc_run;
+            }
+            __syncthreads();
+            if (t == 0) {
+                o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+                o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+            }
+            acc_run = 0.0;
+            dc_run = 0.0;
+        } else if (t == 0) {
+            o[0] = 0.0;
+            o[1] = 0.0;
         }
     }
 }

@@ -75,6 +75,14 @@ template <int S> __device__ __forceinline__ float2 tw(const Tw2 &t, uint32_t idx
     return S > 0 ? w : cconj(w);
 }
 
+// The same table at an index that is a multiple of 2^lo_bits (the low factor is
+// exactly 1): hi[i] for idx = i << lo_bits, no product.
+template <int S> __device__ __forceinline__ float2 tw_hi(const Tw2 &t, uint32_t i) {
+    const F2v h = ((const ConstF2 *)t.hi)[i];
+    const float2 w = make_float2(h.x, h.y);
+    return S > 0 ? w : cconj(w);
+}
+
 // Single-level table of exp(+2*pi*i*m/16384), m = 0..16383 (FP64 -> FP32).
 template <int S> __device__ __forceinline__ float2 tw1(const float2 *T, uint32_t idx) {
     const float2 w = T[idx];

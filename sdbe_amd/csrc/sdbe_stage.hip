@@ -424,7 +424,7 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             // e^{-2 pi i (t + 1024 q) k1 / M'} = A(t) * B(q); B is uniform over the workgroup
             const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
 #pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<-1>(tb.nmp, (uint32_t)(1024 * q * k1))));
+            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw_hi<-1>(tb.nmp, (uint32_t)(q * k1))));
             fft16k_p123<-1>(v, lds, &ftw, t);
             // the next row (unconditionally: the last row loads itself again, so
             // the registers are always overwritten); keep the loads here, the
@@ -457,7 +457,7 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             __syncthreads();
             float2 Br[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) Br[j] = tw<-1>(tb.nmp, (uint32_t)(1024 * (4 * kb + j) * k1));
+            for (int j = 0; j < 4; ++j) Br[j] = tw_hi<-1>(tb.nmp, (uint32_t)((4 * kb + j) * k1));
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0, t) & 1023) * k1));
@@ -882,10 +882,10 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
 #pragma unroll
             for (int q = 0; q < 16; ++q) v[q] = lds_get(lds, lds_re(tr) + 2048 * q);
             // inter-pass twiddle e^{2 pi i c tx / 2^20}, c = t + 1024 q: per-thread factor
-            // times a workgroup-uniform one
+            // times a workgroup-uniform one, e^{2 pi i q tx / 1024} = the high table's entry q tx
             const float2 A = tw<+1>(tb.np, (uint32_t)(tr * tx));
 #pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw<+1>(tb.np, (uint32_t)(1024 * q * tx))));
+            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw_hi<+1>(tb.np, (uint32_t)(q * tx))));
             fft16k_p123<+1>(v, lds, &ftw, tr);
             c2_load(Bc, next < n_rows ? next : row, pr, pi);
             // keep the loads here: the scheduler would otherwise sink them below
@@ -897,7 +897,7 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             const int t = row_tid(), l = t & 63, kb = l >> 4;
             float2 Br[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) Br[j] = tw<+1>(tb.np, (uint32_t)(1024 * (4 * kb + j) * tx));
+            for (int j = 0; j < 4; ++j) Br[j] = tw_hi<+1>(tb.np, (uint32_t)((4 * kb + j) * tx));
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int e0 = mfma_elem(g, 0, t);

@@ -83,11 +83,6 @@ template <int S> __device__ __forceinline__ float2 tw_hi(const Tw2 &t, uint32_t 
     return S > 0 ? w : cconj(w);
 }
 
-// Single-level table of exp(+2*pi*i*m/16384), m = 0..16383 (FP64 -> FP32).
-template <int S> __device__ __forceinline__ float2 tw1(const float2 *T, uint32_t idx) {
-    const float2 w = T[idx];
-    return S > 0 ? w : cconj(w);
-}
 
 struct Tables {
     const float2 *fft;    // FftTw image (640 entries) for the 16384-point FFT

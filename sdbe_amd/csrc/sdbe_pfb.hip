@@ -328,14 +328,12 @@ k_pfb_scan(const double *__restrict__ pst, const double *__restrict__ mt, int ns
 
 template <bool GEN>
 __device__ __forceinline__ void ffchunk(int c0, int s1, const float (&y)[SEG], double a0, double a1, double a2,
-                                        double a3, Row &r1, Row &r2, Row &r3, double (&st)[3],
-                                        double *__restrict__ uc) {
+                                        double a3, Row &r1, Row &r2, Row &r3, double (&st)[3]) {
 #pragma unroll
     for (int k = 0; k < SEG; ++k) {
         if (!GEN || c0 + k < s1) {
             const Row r = chol_row<GEN>(c0 + k, a0, a1, a2, a3, r1, r2, r3);
             const double ui = ((double)y[k] - r.d1 * st[0] - r.d2 * st[1] - r.d3 * st[2]) * r.inv0;
-            uc[(size_t)k * LB] = ui;
             st[2] = st[1];
             st[1] = st[0];
             st[0] = ui;
@@ -346,10 +344,13 @@ __device__ __forceinline__ void ffchunk(int c0, int s1, const float (&y)[SEG], d
     }
 }
 
-// Forward, final pass: the span from its true incoming state, u[i][n].
+// Forward, final pass: the span from its true incoming state.  u itself is not
+// stored: the state entering every 16-row chunk, uck[c][3][LB] = (u[16c-1],
+// u[16c-2], u[16c-3]), is, and the backward passes recompute a chunk's u from it
+// and yh along with the chunk's factor rows.
 __global__ void __launch_bounds__(SOLVE_WG)
 k_pfb_ffin(const float *__restrict__ yh, int nblock, const double *__restrict__ a,
-           const double *__restrict__ ckpt, const double *__restrict__ state, double *__restrict__ u) {
+           const double *__restrict__ ckpt, const double *__restrict__ state, double *__restrict__ uck) {
     const int n = blockIdx.x * SOLVE_WG + threadIdx.x, sp = blockIdx.y, s0 = sp * SPAN;
     const int s1 = min(s0 + SPAN, nblock);
     double a0, a1, a2, a3;
@@ -358,15 +359,19 @@ k_pfb_ffin(const float *__restrict__ yh, int nblock, const double *__restrict__ 
     ckpt_rows(load_ckpt(ckpt, s0 / SEG, n), r1, r2, r3);
     const double *st = state + (size_t)sp * 3 * LB + n;
     double st3[3] = {st[0], st[(size_t)LB], st[(size_t)2 * LB]};   // u[i-1], u[i-2], u[i-3]
-    for (int c0 = s0; c0 < s1; c0 += SEG) {
+    for (int c0 = s0;; c0 += SEG) {
+        double *uc = uck + (size_t)(c0 / SEG) * 3 * LB + n;
+        uc[0] = st3[0];
+        uc[(size_t)LB] = st3[1];
+        uc[(size_t)2 * LB] = st3[2];
+        if (c0 + SEG >= s1) break;   // the span's last chunk feeds no later chunk
         float y[SEG];
 #pragma unroll
-        for (int k = 0; k < SEG; ++k) y[k] = yh[(size_t)min(c0 + k, nblock - 1) * LB + n];
-        double *uc = u + (size_t)c0 * LB + n;
-        if (c0 >= SEG && c0 + SEG <= s1)
-            ffchunk<false>(c0, s1, y, a0, a1, a2, a3, r1, r2, r3, st3, uc);
+        for (int k = 0; k < SEG; ++k) y[k] = yh[(size_t)(c0 + k) * LB + n];
+        if (c0 >= SEG)
+            ffchunk<false>(c0, s1, y, a0, a1, a2, a3, r1, r2, r3, st3);
         else
-            ffchunk<true>(c0, s1, y, a0, a1, a2, a3, r1, r2, r3, st3, uc);
+            ffchunk<true>(c0, s1, y, a0, a1, a2, a3, r1, r2, r3, st3);
     }
 }
 
@@ -386,16 +391,35 @@ __device__ __forceinline__ void chunk_rows(const Ckpt &ck, int row0, double a0, 
     }
 }
 
+// The chunk's u (forward substitution from the state entering the chunk, with
+// the factor rows just recomputed), for the backward passes.
+__device__ __forceinline__ void chunk_u(const Row (&Lx)[SEG + 3], const float (&y)[SEG], const double (&us)[3],
+                                        double (&uu)[SEG]) {
+    double w1 = us[0], w2 = us[1], w3 = us[2];
+#pragma unroll
+    for (int k = 0; k < SEG; ++k) {
+        const double ui = ((double)y[k] - Lx[k].d1 * w1 - Lx[k].d2 * w2 - Lx[k].d3 * w3) * Lx[k].inv0;
+        uu[k] = ui;
+        w3 = w2;
+        w2 = w1;
+        w1 = ui;
+    }
+}
+
 template <bool GEN, int NC, bool HOM>
-__device__ __forceinline__ void bchunk(const Ckpt &ck, int c0, int s1, const double (&uu)[SEG], double a0, double a1,
-                                       double a2, double a3, Row (&Lx)[SEG + 3], double (&st)[NC][3]) {
+__device__ __forceinline__ void bchunk(const Ckpt &ck, int c0, int s1, const float (&y)[SEG], const double (&us)[3],
+                                       double a0, double a1, double a2, double a3, Row (&Lx)[SEG + 3],
+                                       double (&st)[NC][3]) {
     chunk_rows<SEG, GEN>(ck, c0, a0, a1, a2, a3, Lx);
+    double uu[SEG];
+    if constexpr (!HOM) chunk_u(Lx, y, us, uu);
 #pragma unroll
     for (int k = SEG - 1; k >= 0; --k) {
         if (!GEN || c0 + k < s1) {
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
-                const double rhs = HOM ? 0.0 : uu[k];
+                double rhs = 0.0;
+                if constexpr (!HOM) rhs = uu[k];
                 const double zi = (rhs - Lx[k + 1].d1 * st[j][0] - Lx[k + 2].d2 * st[j][1] -
                                    Lx[k + 3].d3 * st[j][2]) * Lx[k].inv0;
                 st[j][2] = st[j][1];
@@ -416,7 +440,7 @@ __device__ __forceinline__ void bchunk(const Ckpt &ck, int c0, int s1, const dou
 // Lx[16..18].  The next chunk's checkpoint is loaded one chunk ahead.
 template <bool HOM>
 __global__ void __launch_bounds__(SOLVE_WG)
-k_pfb_bspan(const double *__restrict__ u, int nblock, const double *__restrict__ a,
+k_pfb_bspan(const float *__restrict__ yh, const double *__restrict__ uck, int nblock, const double *__restrict__ a,
             const double *__restrict__ ckpt, double *__restrict__ out) {
     constexpr int NC = HOM ? 3 : 1;
     const int n = blockIdx.x * SOLVE_WG + threadIdx.x, sp = blockIdx.y, s0 = sp * SPAN;
@@ -431,24 +455,33 @@ k_pfb_bspan(const double *__restrict__ u, int nblock, const double *__restrict__
     chains_init<NC>(st);
     for (int c = chi; c >= clo; --c) {
         const int c0 = c * SEG;
-        double uu[SEG];
+        float y[SEG];
+        double us[3] = {0.0, 0.0, 0.0};
+        if constexpr (!HOM) {
 #pragma unroll
-        for (int k = 0; k < SEG; ++k) uu[k] = HOM ? 0.0 : u[(size_t)min(c0 + k, nblock - 1) * LB + n];
+            for (int k = 0; k < SEG; ++k) y[k] = yh[(size_t)min(c0 + k, nblock - 1) * LB + n];
+            const double *uc = uck + (size_t)c * 3 * LB + n;
+            us[0] = uc[0];
+            us[1] = uc[(size_t)LB];
+            us[2] = uc[(size_t)2 * LB];
+        }
         const Ckpt ckn = load_ckpt(ckpt, max(c - 1, clo), n);
         if (c0 >= SEG && c0 + SEG <= s1)
-            bchunk<false, NC, HOM>(ck, c0, s1, uu, a0, a1, a2, a3, Lx, st);
+            bchunk<false, NC, HOM>(ck, c0, s1, y, us, a0, a1, a2, a3, Lx, st);
         else
-            bchunk<true, NC, HOM>(ck, c0, s1, uu, a0, a1, a2, a3, Lx, st);
+            bchunk<true, NC, HOM>(ck, c0, s1, y, us, a0, a1, a2, a3, Lx, st);
         ck = ckn;
     }
     chains_store<NC>(out, sp, n, st);
 }
 
 template <bool GEN>
-__device__ __forceinline__ void bfchunk(const Ckpt &ck, int c0, int s1, const double (&uu)[SEG], double a0,
-                                        double a1, double a2, double a3, const double (&w)[4], Row (&Lx)[SEG + 3],
-                                        double (&z)[3], float *__restrict__ xc) {
+__device__ __forceinline__ void bfchunk(const Ckpt &ck, int c0, int s1, const float (&y)[SEG], const double (&us)[3],
+                                        double a0, double a1, double a2, double a3, const double (&w)[4],
+                                        Row (&Lx)[SEG + 3], double (&z)[3], float *__restrict__ xc) {
     chunk_rows<SEG, GEN>(ck, c0, a0, a1, a2, a3, Lx);
+    double uu[SEG];
+    chunk_u(Lx, y, us, uu);
 #pragma unroll
     for (int k = SEG - 1; k >= 0; --k) {
         if (!GEN || c0 + k < s1) {
@@ -467,7 +500,7 @@ __device__ __forceinline__ void bfchunk(const Ckpt &ck, int c0, int s1, const do
 // 4-tap projection x = P^T z: x[i + 3] = w0 z[i+3] + w1 z[i+2] + w2 z[i+1] + w3 z[i]
 // for the span's rows i; span 0 also emits x[0..2] (z[i] = 0 for i < 0).
 __global__ void __launch_bounds__(SOLVE_WG)
-k_pfb_bfin(const double *__restrict__ u, int nblock, const double *__restrict__ a,
+k_pfb_bfin(const float *__restrict__ yh, const double *__restrict__ uck, int nblock, const double *__restrict__ a,
            const double *__restrict__ ckpt, const double *__restrict__ state, const float *__restrict__ win,
            float *__restrict__ x) {
     const int n = blockIdx.x * SOLVE_WG + threadIdx.x, sp = blockIdx.y, s0 = sp * SPAN;
@@ -483,15 +516,17 @@ k_pfb_bfin(const double *__restrict__ u, int nblock, const double *__restrict__ 
     double z[3] = {st[0], st[(size_t)LB], st[(size_t)2 * LB]};   // z[i+1], z[i+2], z[i+3]
     for (int c = chi; c >= clo; --c) {
         const int c0 = c * SEG;
-        double uu[SEG];
+        float y[SEG];
 #pragma unroll
-        for (int k = 0; k < SEG; ++k) uu[k] = u[(size_t)min(c0 + k, nblock - 1) * LB + n];
+        for (int k = 0; k < SEG; ++k) y[k] = yh[(size_t)min(c0 + k, nblock - 1) * LB + n];
+        const double *uc = uck + (size_t)c * 3 * LB + n;
+        const double us[3] = {uc[0], uc[(size_t)LB], uc[(size_t)2 * LB]};
         const Ckpt ckn = load_ckpt(ckpt, max(c - 1, clo), n);
         float *xc = x + (size_t)(c0 + 3) * LB + n;
         if (c0 >= SEG && c0 + SEG <= s1)
-            bfchunk<false>(ck, c0, s1, uu, a0, a1, a2, a3, w, Lx, z, xc);
+            bfchunk<false>(ck, c0, s1, y, us, a0, a1, a2, a3, w, Lx, z, xc);
         else
-            bfchunk<true>(ck, c0, s1, uu, a0, a1, a2, a3, w, Lx, z, xc);
+            bfchunk<true>(ck, c0, s1, y, us, a0, a1, a2, a3, w, Lx, z, xc);
         ck = ckn;
     }
     if (sp == 0) {
@@ -512,7 +547,7 @@ struct sdbe_pfb {
     float *win = nullptr;         // [4][32768]
     double *a = nullptr;          // [4][32768] band of P P^T per branch
     float *yh = nullptr;          // [max_spec][32768]
-    double *u = nullptr;          // [max_spec][32768]
+    double *uck = nullptr;        // [chunks][3][32768] forward state entering every 16-row chunk
     double *ckpt = nullptr;       // [chunks][6][32768] factor state at every 16th row
     double *pst = nullptr;        // [spans][3][32768] particular outgoing states
     double *mf = nullptr;         // [spans][9][32768] forward transfers of full spans
@@ -543,7 +578,7 @@ void sdbe_pfb_destroy(sdbe_pfb_t *p) {
     (void)hipFree(p->win);
     (void)hipFree(p->a);
     (void)hipFree(p->yh);
-    (void)hipFree(p->u);
+    (void)hipFree(p->uck);
     (void)hipFree(p->ckpt);
     (void)hipFree(p->pst);
     (void)hipFree(p->mf);
@@ -596,7 +631,7 @@ int sdbe_pfb_create(int device, int32_t n_taps, const float *h_window, int32_t m
     PALLOC(p->win, (size_t)NT * LB * sizeof(float));
     PALLOC(p->a, (size_t)NT * LB * sizeof(double));
     PALLOC(p->yh, (size_t)max_spectra * LB * sizeof(float));
-    PALLOC(p->u, (size_t)max_spectra * LB * sizeof(double));
+    PALLOC(p->uck, nck * 3 * LB * sizeof(double));
     PALLOC(p->ckpt, nck * CKW * LB * sizeof(double));
     PALLOC(p->pst, nspan * 3 * LB * sizeof(double));
     PALLOC(p->mf, nspan * 9 * LB * sizeof(double));
@@ -615,8 +650,9 @@ int sdbe_pfb_create(int device, int32_t n_taps, const float *h_window, int32_t m
                        p->ckpt);
     hipLaunchKernelGGL(k_pfb_fspan<true>, dim3(LB / SOLVE_WG, nspan), dim3(SOLVE_WG), 0, 0, (const float *)nullptr,
                        (int)(nspan * SPAN), (const double *)p->a, (const double *)p->ckpt, p->mf);
-    hipLaunchKernelGGL(k_pfb_bspan<true>, dim3(LB / SOLVE_WG, nspan), dim3(SOLVE_WG), 0, 0, (const double *)nullptr,
-                       (int)(nspan * SPAN), (const double *)p->a, (const double *)p->ckpt, p->mb);
+    hipLaunchKernelGGL(k_pfb_bspan<true>, dim3(LB / SOLVE_WG, nspan), dim3(SOLVE_WG), 0, 0, (const float *)nullptr,
+                       (const double *)nullptr, (int)(nspan * SPAN), (const double *)p->a, (const double *)p->ckpt,
+                       p->mb);
     if (hipDeviceSynchronize() != hipSuccess) {
         rc = SDBE_ERR_HIP;
         goto fail;
@@ -654,13 +690,13 @@ int sdbe_pfb_inverse(sdbe_pfb_t *p, const void *d_spec, int32_t n_spectra, float
     hipLaunchKernelGGL(k_pfb_scan, dim3(LB / SOLVE_WG), dim3(SOLVE_WG), 0, s, (const double *)p->pst, (const double *)p->mf, nspan, 1,
                        p->state);
     hipLaunchKernelGGL(k_pfb_ffin, grid, dim3(SOLVE_WG), 0, s, (const float *)p->yh, (int)n_spectra, a, ck,
-                       (const double *)p->state, p->u);
-    hipLaunchKernelGGL(k_pfb_bspan<false>, grid, dim3(SOLVE_WG), 0, s, (const double *)p->u, (int)n_spectra, a, ck,
-                       p->pst);
+                       (const double *)p->state, p->uck);
+    hipLaunchKernelGGL(k_pfb_bspan<false>, grid, dim3(SOLVE_WG), 0, s, (const float *)p->yh,
+                       (const double *)p->uck, (int)n_spectra, a, ck, p->pst);
     hipLaunchKernelGGL(k_pfb_scan, dim3(LB / SOLVE_WG), dim3(SOLVE_WG), 0, s, (const double *)p->pst, (const double *)p->mb, nspan, -1,
                        p->state);
-    hipLaunchKernelGGL(k_pfb_bfin, grid, dim3(SOLVE_WG), 0, s, (const double *)p->u, (int)n_spectra, a, ck,
-                       (const double *)p->state, (const float *)p->win, d_x);
+    hipLaunchKernelGGL(k_pfb_bfin, grid, dim3(SOLVE_WG), 0, s, (const float *)p->yh, (const double *)p->uck,
+                       (int)n_spectra, a, ck, (const double *)p->state, (const float *)p->win, d_x);
     PFB_TRY(p, hipGetLastError());
     return SDBE_OK;
 }

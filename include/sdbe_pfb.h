@@ -35,7 +35,7 @@ extern "C" {
 typedef struct sdbe_pfb sdbe_pfb_t;
 
 /* h_window: host float [n_taps * 32768]; n_taps must be 4.  max_spectra
- * bounds the spectra per call (scratch of about 16 bytes per sample).  The
+ * bounds the spectra per call (scratch of about 9 bytes per sample).  The
  * plan precomputes the window's banded factor checkpoints and span
  * transfers at creation (under a millisecond for 4096 spectra).           */
 int sdbe_pfb_create(int device, int32_t n_taps, const float *h_window, int32_t max_spectra,

@@ -297,6 +297,36 @@ def test_process_group_packet_stage_a_matches_rows(torch_cuda, one_group):
     ref.close()
 
 
+def test_process_group_missing_frames_and_empty(torch_cuda, one_group):
+    """Whole B-frames missing (frames 5 and 6: blocks 16 and 19 straddle them)
+    and a group with no packets at all: sdbe_process_group equals stage A over
+    the int8 rows of the same input (zeros where nothing arrived), byte for byte."""
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    _, pk_shuf, b0 = one_group
+    w5 = pk_shuf[:, 5]
+    bc = ((w5 >> 24) + ((pk_shuf[:, 4] & 0xFFFFFF) << 8)).astype(np.int64) - b0
+    pk_cut = np.ascontiguousarray(pk_shuf[(bc != 5) & (bc != 6)])
+    st = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD)
+    ref = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD)
+    d = st.to_device_packets(pk_cut)
+    _say("[group-miss] gpu process_group without frames 5, 6 (n_pkts=%d)" % pk_cut.shape[0])
+    out = st.process_group(d, pk_cut.shape[0], b0).cpu().numpy()
+    g0, g1, _ = ref.unpack_int8(d, pk_cut.shape[0], b0)
+    for pol, g in enumerate((g0, g1)):
+        want = ref.process_spectra(g.reshape(128, 39, 16384, 2).contiguous()).cpu().numpy().reshape(-1)
+        assert np.array_equal(out[pol].reshape(-1), want)
+    _say("[group-miss] gpu process_group with no packets")
+    empty = torch.zeros(1056, dtype=torch.uint8, device="cuda")
+    out0 = st.process_group(empty, 0, b0).cpu().numpy()
+    assert st.stats()["n_unique"] == 0
+    want0 = ref.process_spectra(torch.zeros((128, 39, 16384, 2), dtype=torch.int8, device="cuda")).cpu().numpy()
+    for pol in (0, 1):
+        assert np.array_equal(out0[pol].reshape(-1), want0.reshape(-1))
+    st.close()
+    ref.close()
+
+
 def test_process_group_unaligned_packets(torch_cuda, one_group):
     """A packet buffer that is only 4-byte aligned takes the int8-row path
     (stage A's packet reads are 8-byte words); the output is the same."""

@@ -925,9 +925,11 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             for (int q = 0; q < 16; ++q) {
                 const float a = v[q].x * scale - to.y;
                 const float b = v[q].y * scale - to.y;
-                const uint32_t ca = ((((fabsf(a) >= to.x) ? 1u : 0u) | 2u) ^ (a < 0.f ? 3u : 0u)) & 3u;
-                const uint32_t cb = ((((fabsf(b) >= to.x) ? 1u : 0u) | 2u) ^ (b < 0.f ? 3u : 0u)) & 3u;
-                dst[1024 * q] = (uint8_t)(ca | (cb << 2));
+                // code = (|v| >= th ? 3 : 2) ^ (v < 0 ? 3 : 0): 3 if v >= th, 2 if 0 <= v < th,
+                // 1 if -th < v < 0, 0 if v <= -th; b's code in bits 2-3 (disjoint: XOR = OR)
+                const uint32_t pair = ((fabsf(a) >= to.x) ? 3u : 2u) ^ (a < 0.f ? 3u : 0u) ^
+                                      ((fabsf(b) >= to.x) ? 12u : 8u) ^ (b < 0.f ? 12u : 0u);
+                dst[1024 * q] = (uint8_t)pair;
             }
         }
         if (next >= n_rows) break;

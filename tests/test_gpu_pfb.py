@@ -112,6 +112,26 @@ def test_small_batches_and_errors(torch_cuda):
     p.close()
 
 
+def test_span_boundaries(torch_cuda):
+    """The banded solve runs in 256-row spans joined by a scan: one full span,
+    a full span plus one row, two full spans, and two plus one row (a last span
+    of one row), on a plan whose capacity (600) is not a span multiple."""
+    torch = torch_cuda
+    from sdbe_amd.pfb import Pfb
+    w32 = _window("hann_sinc")
+    rng = np.random.default_rng(14)
+    p = Pfb(w32, max_spectra=600)
+    for n in (256, 257, 512, 513):
+        spec = (rng.normal(size=(n, 16385)) + 1j * rng.normal(size=(n, 16385))).astype(np.complex64)
+        _say("[pfb] inverse n=%d (span boundaries)" % n)
+        got = p.inverse(torch.from_numpy(spec).cuda()).cpu().numpy()
+        want = O.pfb_inverse(spec.astype(np.complex128), w32.astype(np.float64))
+        snr = _snr(want, got)
+        _say("[pfb] inverse n=%d: %.1f dB" % (n, snr))
+        assert snr >= 100.0
+    p.close()
+
+
 def test_config5_reconstruction_snr(torch_cuda):
     """4096 spectra from a forward PFB of white Gaussian noise plus a linear
     chirp, each channel's re / im quantised to 2 bits (thresholds 0, +-0.9816

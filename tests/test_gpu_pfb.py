@@ -74,7 +74,7 @@ def test_inverse_matches_oracle(torch_cuda, kind):
     from sdbe_amd.pfb import Pfb
     w32 = _window(kind)
     rng = np.random.default_rng(12)
-    n = 200                                             # not a multiple of the 32-row segments
+    n = 200                                             # not a multiple of the solve's 16-row chunks
     x = rng.normal(size=(n + 3) * LB)
     spec = O.pfb_forward(x, w32.astype(np.float64)).astype(np.complex64)
     p = Pfb(w32, max_spectra=256)

@@ -387,8 +387,90 @@ __device__ __forceinline__ int row_tid() {
     return t;
 }
 
+// fft16k_p123 = fft16k_p12 (passes 1 and 2, ending with the pass-2 outputs in
+// LDS behind a barrier) + fft16k_p3 (pass 3, then the pass-4 inputs read into v).
+template <int S>
+__device__ __forceinline__ void fft16k_p12(float2 *v, float *lds, const FftTw *tw, int t);
+template <int S>
+__device__ __forceinline__ void fft16k_p3(float2 *v, float *lds, const FftTw *tw, int t);
+
 template <int S>
 __device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *tw, int t) {
+    fft16k_p12<S>(v, lds, tw, t);
+    fft16k_p3<S>(v, lds, tw, t);
+}
+
+// The same transform with a wave-local third exchange: fft16k_p12, then
+// fft16k_p3w (pass 3 and the exchange into pass 4, no workgroup barrier), then
+// fft16k_p4 with t' = fft16k_tord(t).  Thread t ends with element
+// fft16k_tord(t) + 1024 q in v[q].
+//
+// Thread t runs the pass-3 butterfly m = T = fft16k_tord(t) and the pass-4
+// butterflies T + 1024 rho (rho = 0..3).  Every element that wave w reads or
+// writes in passes 3 and 4 has bits 4..7 equal to w: the pass-3 inputs
+// m + 1024 r, the pass-3 outputs 4096 (m >> 8) + (m & 255) + 256 r, and the
+// pass-4 inputs T + 1024 q.  So the wave writes its pass-3 outputs into the
+// LDS words it read its inputs from, and reads its pass-4 inputs back from
+// them, with no other wave touching those words.  Element o of that exchange
+// sits at lds_re(lds_fold(o)), where lds_fold XORs bits 12-13 into bits 8-9.
+// The writes spread a wave's lanes over element bits 0-3 and 12-13, and the
+// reads over bits 0-3 and 8-9, so both reach all 64 banks (and the 32 of each
+// half-wave), as the pass-3 reads of lds_re(T + 1024 r) do.
+__host__ __device__ constexpr int fft16k_tord(int t) {
+    return 16 * (t >> 6) + (t & 15) + 256 * ((t >> 4) & 3);
+}
+__host__ __device__ constexpr int lds_fold(int o) { return o ^ ((o >> 4) & 0x300); }
+
+// Orders this wave's LDS writes before its later LDS reads of words written by
+// other lanes of the wave (a wave's LDS operations execute in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// pass 3 twiddles of butterfly k = m & 255: w^r, w = e^{S 2 pi i k / 4096}
+template <int S>
+__device__ __forceinline__ void fft16k_tw3(float2 *v, const FftTw *tw, int k) {
+    const int m1 = k, m2 = 2 * k, m4 = 4 * k, m8 = 8 * k;   // < 4096
+    twiddle16_pow(v, ctw<S>(cmul(tw->h4k[m1 >> 6], tw->l4k[m1 & 63])),
+                  ctw<S>(cmul(tw->h4k[m2 >> 6], tw->l4k[m2 & 63])),
+                  ctw<S>(cmul(tw->h4k[m4 >> 6], tw->l4k[m4 & 63])),
+                  ctw<S>(cmul(tw->h4k[m8 >> 6], tw->l4k[m8 & 63])));
+}
+
+template <int S>
+__device__ __forceinline__ void fft16k_p3w(float2 *v, float *lds, const FftTw *tw, int t) {
+    const int T = fft16k_tord(t), k = T & 255;
+    const int rd = lds_re(T);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = lds_get(lds, rd + 2048 * r);
+    fft16k_tw3<S>(v, tw, k);
+    dft16<S>(v);
+    // lds_re passes element bits 10-13 through (as 2 (e & ~1023)), so with
+    // rt = lds_re(T) < 2048 every word below is a per-thread XOR base plus a
+    // constant that fits the DS offset field.
+    // Output r = 4 h + rho is element 4096 b + 1024 h + 256 rho + k (b = T >> 8),
+    // folded to 4096 b + 1024 h + 256 (rho ^ b) + k: word
+    // (rt ^ lds_re(256 rho)) + 8192 b + 2048 h.
+    {
+        const int rt = lds_opaque(rd), hb = 8192 * (T >> 8);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) lds_put(lds, ((rt ^ lds_re(256 * (r & 3))) + hb) + 2048 * (r >> 2), v[r]);
+    }
+    wave_lds_sync();
+    // Element T + 1024 q, q = 4 g + rho, folded to (T ^ 256 g) + 1024 rho + 4096 g:
+    // word (rt ^ lds_re(256 g)) + 8192 g + 2048 rho.
+    {
+        const int rt = lds_opaque(rd);
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            v[q] = lds_get(lds, ((rt ^ lds_re(256 * (q >> 2))) + 8192 * (q >> 2)) + 2048 * (q & 3));
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void fft16k_p12(float2 *v, float *lds, const FftTw *tw, int t) {
     const int rd = lds_re(t);
     // pass 1: radix 16, Ns = 1 (no twiddle)
     dft16<S>(v);
@@ -413,16 +495,17 @@ __device__ __forceinline__ void fft16k_p123(float2 *v, float *lds, const FftTw *
         for (int r = 0; r < 16; ++r) lds_put(lds, base ^ lds_re(16 * r), v[r]);
         __syncthreads();
     }
+}
+
+template <int S>
+__device__ __forceinline__ void fft16k_p3(float2 *v, float *lds, const FftTw *tw, int t) {
+    const int rd = lds_re(t);
     // pass 3: radix 16, Ns = 256
     {
         const int k = t & 255;
 #pragma unroll
         for (int r = 0; r < 16; ++r) v[r] = lds_get(lds, rd + 2048 * r);
-        const int m1 = k, m2 = 2 * k, m4 = 4 * k, m8 = 8 * k;   // < 4096
-        twiddle16_pow(v, ctw<S>(cmul(tw->h4k[m1 >> 6], tw->l4k[m1 & 63])),
-                      ctw<S>(cmul(tw->h4k[m2 >> 6], tw->l4k[m2 & 63])),
-                      ctw<S>(cmul(tw->h4k[m4 >> 6], tw->l4k[m4 & 63])),
-                      ctw<S>(cmul(tw->h4k[m8 >> 6], tw->l4k[m8 & 63])));
+        fft16k_tw3<S>(v, tw, k);
         dft16<S>(v);
         __syncthreads();
         const int base = lds_opaque(lds_re((t >> 8) * 4096 + k));

@@ -425,13 +425,16 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             const float2 A = tw<-1>(tb.nmp, (uint32_t)(t * k1));
 #pragma unroll
             for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw_hi<-1>(tb.nmp, (uint32_t)(q * k1))));
-            fft16k_p123<-1>(v, lds, &ftw, t);
+            // the second transform's third exchange is wave-local: the row comes
+            // out in the order of fft16k_tord (element fft16k_tord(t) + 1024 q in v[q])
+            fft16k_p12<-1>(v, lds, &ftw, t);
+            fft16k_p3w<-1>(v, lds, &ftw, t);
             // the next row (unconditionally: the last row loads itself again, so
             // the registers are always overwritten); keep the loads here, the
             // scheduler would otherwise sink them below this row's stores
             b_load(V, nbpl, nk1, nx);
             __builtin_amdgcn_sched_barrier(0);
-            fft16k_p4<-1>(v, &ftw, t);
+            fft16k_p4<-1>(v, &ftw, fft16k_tord(row_tid()));
         } else {
             // the MFMA path runs both transforms with the inverse-sign operands: the
             // forward FFT is conj(IFFT(conj(x)))
@@ -471,7 +474,7 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             for (int q = 0; q < 16; ++q) v[q] = cconj(v[q]);
         }
         // row-major [k1][k2]: Zf[k1 + 39 k2] lives at k1 * 16384 + k2
-        float2 *dst = Zf + ((size_t)bpl * NBLK + k1) * NCH + t;
+        float2 *dst = Zf + ((size_t)bpl * NBLK + k1) * NCH + (P == 0 ? fft16k_tord(t) : t);
 #pragma unroll
         for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
         if (next >= n_rows) break;
@@ -886,12 +889,15 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             const float2 A = tw<+1>(tb.np, (uint32_t)(tr * tx));
 #pragma unroll
             for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cmul(A, tw_hi<+1>(tb.np, (uint32_t)(q * tx))));
-            fft16k_p123<+1>(v, lds, &ftw, tr);
+            // third exchange wave-local: the row comes out in the order of
+            // fft16k_tord (element fft16k_tord(t) + 1024 q in v[q])
+            fft16k_p12<+1>(v, lds, &ftw, tr);
+            fft16k_p3w<+1>(v, lds, &ftw, tr);
             c2_load(Bc, next < n_rows ? next : row, pr, pi);
             // keep the loads here: the scheduler would otherwise sink them below
             // the stores at the end of the row, exposing their whole latency
             __builtin_amdgcn_sched_barrier(0);
-            fft16k_p4<+1>(v, &ftw, tr);
+            fft16k_p4<+1>(v, &ftw, fft16k_tord(tr));
         } else {
             // MFMA layout: element c = d + 1024 r, twiddle e^{2 pi i c tx/2^20} = A(d) B(r)
             const int t = row_tid(), l = t & 63, kb = l >> 4;
@@ -911,8 +917,10 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             W.init(t);
             fft16k_mfma<+1, P>(v, lds, &ftw, W, t);
         }
+        // element of v[0]: fft16k_tord(t) on the VALU path, t on the matrix-core path
+        const int te = P == 0 ? fft16k_tord(t) : t;
         if (F32) {
-            GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * t;
+            GAS uint64_t *dst = gptr(reinterpret_cast<uint64_t *>(bpd[bpl].out)) + tx + KHI * te;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const float2 y = cscale(v[q], scale);
@@ -920,7 +928,7 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             }
         } else {
             const float2 to = thr[bpl];
-            uint8_t *dst = stage + (size_t)row * NCH + t;
+            uint8_t *dst = stage + (size_t)row * NCH + te;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const float a = v[q].x * scale - to.y;

@@ -469,6 +469,15 @@ __device__ __forceinline__ void fft16k_p3w(float2 *v, float *lds, const FftTw *t
     }
 }
 
+// The whole transform in that form, for callers that store the result: thread t
+// ends holding element fft16k_tord(t) + 1024 q in v[q].
+template <int S>
+__device__ __forceinline__ void fft16k_t(float2 *v, float *lds, const FftTw *tw) {
+    fft16k_p12<S>(v, lds, tw, threadIdx.x);
+    fft16k_p3w<S>(v, lds, tw, threadIdx.x);
+    fft16k_p4<S>(v, tw, fft16k_tord(threadIdx.x));
+}
+
 template <int S>
 __device__ __forceinline__ void fft16k_p12(float2 *v, float *lds, const FftTw *tw, int t) {
     const int rd = lds_re(t);

@@ -129,11 +129,11 @@ k_pfb_c2r(const float2 *__restrict__ spec, float *__restrict__ yh, const float2 
         const float2 sum = cadd(a, cconj(b)), dif = csub(a, cconj(b));
         v[q] = cadd(sum, mul_si<+1>(cmul(tw32k[k], dif)));
     }
-    fft16k<+1>(v, lds, &ftw);
-    float2 *dst = reinterpret_cast<float2 *>(yh + (size_t)i * LB);
+    fft16k_t<+1>(v, lds, &ftw);   // z[fft16k_tord(t) + 1024 q] in v[q]
+    float2 *dst = reinterpret_cast<float2 *>(yh + (size_t)i * LB) + fft16k_tord(t);
     const float sc = 1.0f / (float)LB;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) dst[t + 1024 * q] = cscale(v[q], sc);
+    for (int q = 0; q < 16; ++q) dst[1024 * q] = cscale(v[q], sc);
 }
 
 // One row of the banded Cholesky factor of the Toeplitz band (a0, a1, a2, a3):

@@ -194,8 +194,8 @@ k_trim_c2r(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb) {
         const float2 sum = cadd(a, cconj(b)), dif = csub(a, cconj(b));
         v[q] = cadd(sum, mul_si<+1>(cmul(tw<+1>(tb.n32k, (uint32_t)k), dif)));
     }
-    fft16k<+1>(v, lds, &ftw);
-    float2 *dst = T + ((size_t)bpl * D + s) * NCH + t;
+    fft16k_t<+1>(v, lds, &ftw);   // element fft16k_tord(t) + 1024 q in v[q]
+    float2 *dst = T + ((size_t)bpl * D + s) * NCH + fft16k_tord(t);
 #pragma unroll
     for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
 }

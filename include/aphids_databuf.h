@@ -1,9 +1,11 @@
 /*
  * aphids_databuf.h -- the hashpipe databuf types the GPU stage exchanges with
  * the APHIDS input and output threads, with byte-identical layouts to the
- * reference declarations (6/11 SWARM rate, 39 B-frames per group):
+ * reference declarations at the SWARM rate preset SWARM_RATE_FRAC (default 6:
+ * the 6/11 rate, 39 B-frames per group):
  *   input side   software/aphids/vdif_in_databuf.h:15-152
  *   output side  software/aphids/vdif_out_databuf.h:11-144
+ *   rate presets software/aphids/sample_rates.h:5-74
  * The layouts are an interface (the other threads of aphids.so share them),
  * so names, field order and widths follow the reference; every size and
  * offset is pinned by a static assertion below.
@@ -26,8 +28,37 @@
 extern "C" {
 #endif
 
+/* ------------------------------------------------------------ rate preset */
+/* SWARM_RATE_FRAC selects the group size (BENG_FRAMES_PER_GROUP =
+ * DECIMATION_FACTOR B-frames) and the output slots per polarisation, as
+ * sample_rates.h does.  The reference compiles 11/11 by default
+ * (sample_rates.h:12-14); here the default is the 6/11 rate of the 64/39
+ * resampler.                                                               */
+#ifndef SWARM_RATE_FRAC
+#define SWARM_RATE_FRAC 6
+#endif
+#if SWARM_RATE_FRAC == 6
+#define DECIMATION_FACTOR 39
+#define EXPANSION_FACTOR 32
+#define VDIF_OUT_PKTS_PER_BLOCK 8192
+#elif SWARM_RATE_FRAC == 8
+#define DECIMATION_FACTOR 13
+#define EXPANSION_FACTOR 8
+#define VDIF_OUT_PKTS_PER_BLOCK 2048
+#elif SWARM_RATE_FRAC == 10
+#define DECIMATION_FACTOR 65
+#define EXPANSION_FACTOR 64
+#define VDIF_OUT_PKTS_PER_BLOCK 8192
+#elif SWARM_RATE_FRAC == 11
+#define DECIMATION_FACTOR 143
+#define EXPANSION_FACTOR 128
+#define VDIF_OUT_PKTS_PER_BLOCK 16384
+#else
+#error "SWARM_RATE_FRAC must be 6, 8, 10 or 11"
+#endif
+
 /* -------------------------------------------------------------- constants */
-#define BENG_FRAMES_PER_GROUP        39      /* DECIMATION_FACTOR at 6/11 */
+#define BENG_FRAMES_PER_GROUP        DECIMATION_FACTOR
 #define VDIF_PER_BENG_FRAME          2048
 #define VDIF_IN_PKT_HEADER_SIZE      32
 #define VDIF_IN_PKT_DATA_SIZE        1024
@@ -36,7 +67,6 @@ extern "C" {
 #define BENG_GROUPS_IN_BUFFER        4
 #define VDIF_OUT_PKT_HEADER_SIZE     32
 #define VDIF_OUT_PKT_DATA_SIZE       8192
-#define VDIF_OUT_PKTS_PER_BLOCK      8192    /* 6/11 rate */
 #define VDIF_CHAN                    2
 #define VDIF_OUT_BUFFER_SIZE         4
 #define VDIF_OUT_FRAMES_PER_SECOND   125000
@@ -142,19 +172,22 @@ typedef struct vdif_out_databuf {
 SDBE_LAYOUT_ASSERT(sizeof(vdif_in_header_t) == 32, "vdif_in_header_t");
 SDBE_LAYOUT_ASSERT(offsetof(vdif_in_header_t, beng) == 16 && offsetof(vdif_in_header_t, edh_psn) == 24, "in hdr");
 SDBE_LAYOUT_ASSERT(sizeof(vdif_in_packet_t) == 1056, "vdif_in_packet_t");
-SDBE_LAYOUT_ASSERT(sizeof(beng_group_vdif_buffer_t) == 84344832ul, "group buffer = 39*2048*1056");
+SDBE_LAYOUT_ASSERT(sizeof(beng_group_vdif_buffer_t) == (size_t)BENG_FRAMES_PER_GROUP * 2048 * 1056,
+                   "group buffer = frames * 2048 * 1056 (84,344,832 at 6/11)");
 SDBE_LAYOUT_ASSERT(sizeof(channel_completion_t) == 1, "channel_completion_t");
 SDBE_LAYOUT_ASSERT(sizeof(beng_frame_completion_t) == 272, "beng_frame_completion_t = 8+2+256 -> 272");
 SDBE_LAYOUT_ASSERT(offsetof(beng_frame_completion_t, cc) == 10, "cc offset");
 SDBE_LAYOUT_ASSERT(offsetof(beng_group_completion_t, bgv_buf_cpu) == 8, "bgc.bgv_buf_cpu");
 SDBE_LAYOUT_ASSERT(offsetof(beng_group_completion_t, gpu_id) == 24, "bgc.gpu_id");
 SDBE_LAYOUT_ASSERT(offsetof(beng_group_completion_t, bfc) == 40, "bgc.bfc");
-SDBE_LAYOUT_ASSERT(offsetof(beng_group_completion_t, vdif_header_template) == 40 + 39 * 272, "bgc.template");
-SDBE_LAYOUT_ASSERT(sizeof(beng_group_completion_t) == 10680, "beng_group_completion_t");
+SDBE_LAYOUT_ASSERT(offsetof(beng_group_completion_t, vdif_header_template) == 40 + BENG_FRAMES_PER_GROUP * 272,
+                   "bgc.template");
+SDBE_LAYOUT_ASSERT(sizeof(beng_group_completion_t) == 72 + BENG_FRAMES_PER_GROUP * 272,
+                   "beng_group_completion_t (10680 at 6/11)");
 SDBE_LAYOUT_ASSERT(sizeof(vdif_out_header_t) == 32, "vdif_out_header_t");
 SDBE_LAYOUT_ASSERT(sizeof(vdif_out_packet_t) == 8224, "vdif_out_packet_t");
-SDBE_LAYOUT_ASSERT(sizeof(vdif_out_data_group_t) == 2ul * 8192 * 8192, "vdif_out_data_group_t");
-SDBE_LAYOUT_ASSERT(sizeof(vdif_out_packet_group_t) == 2ul * 8192 * 8224, "vdif_out_packet_group_t");
+SDBE_LAYOUT_ASSERT(sizeof(vdif_out_data_group_t) == 2ul * VDIF_OUT_PKTS_PER_BLOCK * 8192, "vdif_out_data_group_t");
+SDBE_LAYOUT_ASSERT(sizeof(vdif_out_packet_group_t) == 2ul * VDIF_OUT_PKTS_PER_BLOCK * 8224, "vdif_out_packet_group_t");
 SDBE_LAYOUT_ASSERT(offsetof(quantized_storage_t, bit_depth) == 16, "qs.bit_depth");
 SDBE_LAYOUT_ASSERT(offsetof(quantized_storage_t, memcpy_stream) == 32, "qs.memcpy_stream");
 SDBE_LAYOUT_ASSERT(sizeof(quantized_storage_t) == 72, "quantized_storage_t");

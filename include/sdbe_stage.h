@@ -98,21 +98,27 @@ extern "C" {
 
 /* Resampler.  SDBE_RATE_64_39: the 39-to-64 rational resampler of the
  * prototypes (resample.py fft_batched): a block of 39 snapshots becomes
- * 2,097,152 samples, 64 frames per block-pol.  SDBE_RATE_6_11 / _8_11: the
- * band-trimmed path of the aphids GPU thread compiled for that SWARM rate
- * (vdif_inout_gpu_thread.cu:589-709, plans :269-333, sample_rates.h:21-40):
- * per block of D snapshots, a 32768-point C2R per snapshot, 128 real DFTs of
- * 256 D samples, and from each of them bins skip_chan .. skip_chan + 128 E
- * into a 256 E-point C2R, skip_chan = skip_mhz 1e6 / (SWARM_RATE / 256 D).
- *   6/11: D = 39, E = 32: 2496 -> 2048 Msps, 1,048,576 samples, 32 frames per block-pol;
- *   8/11: D = 13, E = 8:  3328 -> 2048 Msps,   262,144 samples,  8 frames per block-pol.
+ * 2,097,152 samples, 64 frames per block-pol.  SDBE_RATE_6_11 / _8_11 /
+ * _10_11 / _11_11: the band-trimmed path of the aphids GPU thread compiled
+ * for that SWARM rate (vdif_inout_gpu_thread.cu:589-709, plans :269-333,
+ * sample_rates.h:21-59): per block of D = DECIMATION_FACTOR snapshots, a
+ * 32768-point C2R per snapshot, 128 real DFTs of 256 D samples, and from each
+ * of them bins skip_chan .. skip_chan + 128 E into a 256 E-point C2R,
+ * skip_chan = skip_mhz 1e6 / (SWARM_RATE / 256 D), E = EXPANSION_FACTOR:
+ *   6/11:  D = 39,  E = 32:  2496 -> 2048 Msps, 1,048,576 samples,  32 frames per block-pol;
+ *   8/11:  D = 13,  E = 8:   3328 -> 2048 Msps,   262,144 samples,   8 frames;
+ *   10/11: D = 65,  E = 64:  4160 -> 4096 Msps, 2,097,152 samples,  64 frames;
+ *   11/11: D = 143, E = 128: 4576 -> 4096 Msps, 4,194,304 samples, 128 frames.
  * In a band-trimmed handle a "block-pol" of the calls below is D snapshots of
- * one polarisation; sdbe_process_blocks and sdbe_set_output_alignment return
- * SDBE_ERR_STATE.  The 10/11 and 11/11 presets, marked unconfirmed and
- * over-memory in the reference, are not provided.                          */
+ * one polarisation and a group is D B-frames (BENG_FRAMES_PER_GROUP =
+ * DECIMATION_FACTOR, sample_rates.h:67; D * 2048 packets, 128 blocks, 128 E
+ * frames per polarisation); sdbe_process_blocks and sdbe_set_output_alignment
+ * return SDBE_ERR_STATE.                                                    */
 #define SDBE_RATE_64_39 0
 #define SDBE_RATE_6_11  6
 #define SDBE_RATE_8_11  8
+#define SDBE_RATE_10_11 10
+#define SDBE_RATE_11_11 11
 
 typedef struct sdbe_config {
     int32_t thresh_policy;      /* SDBE_THRESH_*                                */
@@ -133,7 +139,7 @@ typedef struct sdbe_config {
                                    passes of stages B and C2                   */
     int32_t rate_frac;          /* resampler: SDBE_RATE_64_39 (0), or the aphids
                                    band-trimmed path of a SWARM rate preset,
-                                   SDBE_RATE_6_11 / SDBE_RATE_8_11             */
+                                   SDBE_RATE_6_11 / _8_11 / _10_11 / _11_11    */
     int32_t skip_mhz;           /* band-trimmed path: MHz skipped at the bottom
                                    of the band (aphids_resampler_init skip_mhz) */
     int32_t reserved[2];
@@ -188,19 +194,20 @@ int sdbe_stream_state_from_template(const void *template_header32,
                                     sdbe_stream_state_t *st,
                                     int32_t *skip_int32, int32_t *skip_frames);
 
-/* Process one group.  d_pkts: n_pkts packets of 1056 bytes (device).
- * b_zero: B-count of the group's first frame (bfc[0].b).  d_out: output,
- * polarisation p at d_out + p * out_pol_stride_bytes, 8192 frames each of
+/* Process one group.  d_pkts: n_pkts packets of 1056 bytes (device), at
+ * most the group's 39 * 2048 (band-trimmed: D * 2048).  b_zero: B-count of
+ * the group's first frame (bfc[0].b).  d_out: output, polarisation p at
+ * d_out + p * out_pol_stride_bytes, 8192 frames (band-trimmed: 128 E) each of
  * 8192 (PAYLOAD) or 8224 (FRAMES) bytes.  d_bitmap (optional, device):
- * uint8 [39][256], bit f of [frame][cid] set when packet (frame, cid, f)
+ * uint8 [frames][256], bit f of [frame][cid] set when packet (frame, cid, f)
  * arrived (the cc[] flags of vdif_in_databuf.h:112-130).                   */
 int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
                        uint64_t b_zero, void *d_out, int64_t out_pol_stride_bytes,
                        uint8_t *d_bitmap, void *stream);
 
 /* Unpack one group into the reference's int8 layout: d_beng0/d_beng1 are
- * int8 [39][128][16384][2 = (im, re)] (163,577,856 bytes each), missing
- * packets zero-filled; d_bitmap optional as above.                          */
+ * int8 [frames][128][16384][2 = (im, re)] (163,577,856 bytes each for 39
+ * frames), missing packets zero-filled; d_bitmap optional as above.         */
 int sdbe_unpack_group_int8(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
                            uint64_t b_zero, int8_t *d_beng0, int8_t *d_beng1,
                            uint8_t *d_bitmap, void *stream);

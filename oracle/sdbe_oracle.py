@@ -922,7 +922,8 @@ def reorder_2015(x):
 # rates (software/aphids/sample_rates.h:12-40, vdif_inout_gpu_thread.h:31-53)
 # ---------------------------------------------------------------------------
 SWARM_RATE_FULL = 4576e6                  # sample_rates.h:17
-TRIM_RATES = {6: (39, 32), 8: (13, 8)}    # SWARM_RATE_FRAC -> (DECIMATION, EXPANSION), sample_rates.h:21-40
+# SWARM_RATE_FRAC -> (DECIMATION_FACTOR, EXPANSION_FACTOR), sample_rates.h:21-59
+TRIM_RATES = {6: (39, 32), 8: (13, 8), 10: (65, 64), 11: (143, 128)}
 TRIM_SCALE = 256                          # FFT_SIZE_SWARM_R2C_R2DBE_C2R_SCALE, vdif_inout_gpu_thread.h:45
 
 

@@ -24,6 +24,12 @@ THRESH_FIXED = 2
 RATE_64_39 = 0
 RATE_6_11 = 6
 RATE_8_11 = 8
+RATE_10_11 = 10
+RATE_11_11 = 11
+# SWARM rate preset -> (DECIMATION_FACTOR D, EXPANSION_FACTOR E) of the
+# band-trimmed path (include/sdbe_stage.h); a group is D B-frames, 128 blocks
+# of D snapshots, E frames of 8192 payload bytes per block-pol
+TRIM_PRESETS = {RATE_6_11: (39, 32), RATE_8_11: (13, 8), RATE_10_11: (65, 64), RATE_11_11: (143, 128)}
 FFT_FP32 = 0
 FFT_BF16X3 = 1
 FFT_BF16 = 2

@@ -50,24 +50,42 @@ def build_stage_lib():
     return LIB
 
 
-def build_hashpipe_plugin():
-    """vdif_inout_gpu_thread plugin + host bookkeeping, against the in-process
-    hashpipe API stand-in (production builds use the real hashpipe headers)."""
-    tmp = HASHPIPE_LIB + ".tmp"
-    srcs = [os.path.join(CSRC, "vdif_inout_gpu_thread.c"), os.path.join(CSRC, "aphids_host.c"),
+# Plugin builds: (library, SWARM_RATE_FRAC, band-trimmed resampler).  The
+# default is the 39 -> 64 resampler at the 6/11 rate; the others are the
+# reference thread's band-trimmed path at each SWARM rate preset (their
+# databuf layouts follow the preset, include/aphids_databuf.h).
+PLUGIN_VARIANTS = [
+    (HASHPIPE_LIB, 6, 0),
+    (os.path.join(HERE, "libsdbe_hashpipe_trim6.so"), 6, 1),
+    (os.path.join(HERE, "libsdbe_hashpipe_trim8.so"), 8, 1),
+    (os.path.join(HERE, "libsdbe_hashpipe_trim10.so"), 10, 1),
+    (os.path.join(HERE, "libsdbe_hashpipe_trim11.so"), 11, 1),
+]
+
+
+def build_hashpipe_plugin(lib=HASHPIPE_LIB, rate=6, trimmed=0):
+    """vdif_inout_gpu_thread plugin, the file-fed input thread and the
+    file-writing output thread, and the host bookkeeping, against the
+    in-process hashpipe API stand-in (production builds use the real hashpipe
+    headers)."""
+    tmp = lib + ".tmp"
+    srcs = [os.path.join(CSRC, "vdif_inout_gpu_thread.c"), os.path.join(CSRC, "vdif_in_file_thread.c"),
+            os.path.join(CSRC, "vdif_out_file_thread.c"), os.path.join(CSRC, "aphids_host.c"),
             os.path.join(CSRC, "aphids_databuf_hip.c"), os.path.join(COMPAT, "hashpipe_compat.c")]
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-    # plain C against the HIP runtime API (the HIP headers need the platform macro)
+    # plain C against the HIP runtime API (the HIP headers need the platform macro);
+    # -Bsymbolic: each variant's calls stay inside it when several are loaded
     _run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-DSWARM_RATE_FRAC=%d" % rate, "-DSDBE_PLUGIN_TRIMMED=%d" % trimmed,
           "-I", INCLUDE, "-I", COMPAT, "-I", os.path.join(rocm, "include"), "-o", tmp, *srcs,
-          "-L", HERE, "-lsdbe", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+          "-L", HERE, "-lsdbe", "-L", os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-Bsymbolic",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"), "-lpthread", "-lm"])
-    os.replace(tmp, HASHPIPE_LIB)
-    return HASHPIPE_LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_all(verbose=False):
-    built = [build_stage_lib(), build_hashpipe_plugin()]
+    built = [build_stage_lib()] + [build_hashpipe_plugin(*v) for v in PLUGIN_VARIANTS]
     if verbose:
         for b in built:
             print("built", os.path.relpath(b, ROOT))

@@ -52,6 +52,23 @@ struct hashpipe_thread_args {
     void *user_data;
 };
 
+/* Status buffer: FITS-style 80-character records "KEYWORD = value", ended by
+ * an END record, as hashpipe's shared status memory; the lock serialises
+ * readers and writers of one process.                                      */
+int hashpipe_status_lock(hashpipe_status_t *s);
+int hashpipe_status_unlock(hashpipe_status_t *s);
+#define hashpipe_status_lock_safe(s)                                         \
+    pthread_cleanup_push((void (*)(void *))hashpipe_status_unlock, (void *)(s)); \
+    hashpipe_status_lock(s);
+#define hashpipe_status_unlock_safe(s) \
+    hashpipe_status_unlock(s);         \
+    pthread_cleanup_pop(0);
+/* hashpipe's fitshead accessors: 1 if the keyword was found, else 0.       */
+int hgeti4(const char *hstring, const char *keyword, int *ival);
+int hgets(const char *hstring, const char *keyword, int lstr, char *string);
+int hputi4(char *hstring, const char *keyword, int ival);
+int hputs(char *hstring, const char *keyword, const char *cval);
+
 int register_hashpipe_thread(hashpipe_thread_desc_t *ptm);
 hashpipe_thread_desc_t *find_hashpipe_thread(const char *name);
 int run_threads(void);

@@ -42,9 +42,12 @@ using namespace sdbe;
 namespace {
 
 constexpr int PKT_WORDS = 264;
-constexpr int NFR = 39;                       // frames per group
+constexpr int NFR = 39;                       // frames per group of the 64/39 resampler
 constexpr int NSNAP_GROUP = NFR * 128;        // 4992 snapshots per group
 constexpr int MAP_N = NFR * 256 * 8;          // 79,872 packet slots per group
+// A band-trimmed handle's group is D B-frames (BENG_FRAMES_PER_GROUP =
+// DECIMATION_FACTOR, sample_rates.h:67): at most 143 (11/11).
+constexpr int NFR_MAX = 143;
 constexpr int MAP_NONE = 0x7FFFFFFF;
 // Stage C1 tiles: columns in residue classes mod 39, 16 per tile
 // (c = a + 39 (16 u + j)), 14 tiles per class, 39 classes.
@@ -65,9 +68,10 @@ __global__ void __launch_bounds__(256) k_fill_i32(int *p, int n, int val) {
 
 // One thread per packet: header words 0, 4, 5 give invalid, B-count, cid, fid
 // (vdif_inout_gpu_thread.cu:353-369,437 restated).  Valid packets inside the
-// 39-frame window claim their map slot with atomicMin -> lowest index wins.
+// group's window of nfr frames claim their map slot with atomicMin -> lowest
+// index wins.
 __global__ void __launch_bounds__(256)
-k_packet_map(const uint32_t *__restrict__ pk, int n, uint32_t bzero32, int *__restrict__ map,
+k_packet_map(const uint32_t *__restrict__ pk, int n, uint32_t bzero32, int nfr, int *__restrict__ map,
              int *__restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool inval = false, range = false, acc = false;
@@ -79,7 +83,7 @@ k_packet_map(const uint32_t *__restrict__ pk, int n, uint32_t bzero32, int *__re
         const int32_t rel = (int32_t)(bc32 - bzero32);
         if (w0 >> 31) {
             inval = true;
-        } else if (rel < 0 || rel >= NFR || fid >= 8u) {
+        } else if (rel < 0 || rel >= nfr || fid >= 8u) {
             range = true;
         } else {
             acc = true;
@@ -100,10 +104,11 @@ k_packet_map(const uint32_t *__restrict__ pk, int n, uint32_t bzero32, int *__re
 // channel offset (3,2,1,0,7,6,5,4)[q]; its bits from the LSB are pol1 im,
 // pol1 re, pol0 im, pol0 re, each decoded as (code & 3) - 2
 // (vdif_inout_gpu_thread.cu:387-392, 464-494).  Output: the reference int8
-// layout [pol][snapshot][16384][im, re] (its beng_0 / beng_1 buffers), a
-// missing packet leaving 0 + 0j; plus the fid bitmap and the unique count.
+// layout [pol][snapshot][16384][im, re] (its beng_0 / beng_1 buffers, nsnap
+// snapshots per polarisation), a missing packet leaving 0 + 0j; plus the fid
+// bitmap and the unique count.
 __global__ void __launch_bounds__(256)
-k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, int8_t *__restrict__ beng,
+k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, int nsnap, int8_t *__restrict__ beng,
           uint8_t *__restrict__ bitmap, int *__restrict__ cnt) {
     __shared__ int pidx[16];
     __shared__ uint32_t pay[16][257];     // packet payload words, [w][payload word]
@@ -149,7 +154,7 @@ k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, int8_t *
         const uint32_t re0 = (uint32_t)(uint8_t)(int8_t)((int)(n0 >> 2) - 2);
         const uint32_t im1 = (uint32_t)(uint8_t)(int8_t)((int)(n1 & 3u) - 2);
         const uint32_t re1 = (uint32_t)(uint8_t)(int8_t)((int)(n1 >> 2) - 2);
-        const size_t row = (size_t)pol * NSNAP_GROUP + (size_t)frame * 128 + sn;
+        const size_t row = (size_t)pol * nsnap + (size_t)frame * 128 + sn;
         uint32_t *dst = reinterpret_cast<uint32_t *>(beng + (row * NCH + (size_t)h * 128 + ch0) * 2);
         *dst = im0 | (re0 << 8) | (im1 << 16) | (re1 << 24);
     }
@@ -1131,6 +1136,8 @@ struct sdbe_handle {
     float2 *trim_tab = nullptr;
     int snaps = NBLK;                // snapshots per block-pol
     int fpb = OUT_FRAMES_BLOCK;      // output frames per block-pol
+    int nfr = NFR;                   // B-frames per group (DECIMATION_FACTOR of the rate)
+    int map_n = MAP_N;               // packet slots per group: nfr * 2048
     // re-framed group output (output-thread time alignment)
     int align_pending = 0;           // the next group starts an aligned stream
     int align_active = 0;
@@ -1214,48 +1221,63 @@ void tw_fill(float2 *host, const TwSpec &s) {
     }
 }
 
-int build_tables(sdbe_handle_t *h) {
-    const TwSpec specs[6] = {
-        {16384, 14, 0},                 // unused slot (the FFT image is stored separately)
-        {32768, 7, 16384},              // n32k
-        {MP, 10, (int64_t)MP},          // nmp: n * k1 < 16384 * 39
-        {2 * (int64_t)MP, 10, (int64_t)MP + 1},   // nm: m <= M'
-        {2 * (int64_t)P20, 10, (int64_t)P20},     // nl: k < 2^20
-        {P20, 10, (int64_t)P20},        // np: c * tx < 2^20, r * j * 16384 < 2^20
-    };
-    size_t total = FFT_TW_N;
-    for (int i = 1; i < 6; ++i) total += tw_entries(specs[i]);
-    const size_t c1col_off = total;
-    total += 2 * NCH;
-    std::vector<float2> host(total);
-    size_t off[6];
+// The resampler's twiddle tables (FFT image, the two-level tables of
+// N = 32768, 638,976, 1,277,952, 2,097,152 and 1,048,576, stage C1's column
+// twiddles), computed in FP64 on the host and stored FP32, back to back.
+constexpr TwSpec kTwSpecs[6] = {
+    {16384, 14, 0},                 // unused slot (the FFT image is stored separately)
+    {32768, 7, 16384},              // n32k
+    {MP, 10, (int64_t)MP},          // nmp: n * k1 < 16384 * 39
+    {2 * (int64_t)MP, 10, (int64_t)MP + 1},   // nm: m <= M'
+    {2 * (int64_t)P20, 10, (int64_t)P20},     // nl: k < 2^20
+    {P20, 10, (int64_t)P20},        // np: c * tx < 2^20, r * j * 16384 < 2^20
+};
+constexpr size_t tw_entries_c(const TwSpec &s) {
+    return ((size_t)1 << s.lo_bits) + (size_t)((s.max_idx >> s.lo_bits) + 2);
+}
+constexpr size_t kTabC1Col = FFT_TW_N + tw_entries_c(kTwSpecs[1]) + tw_entries_c(kTwSpecs[2]) +
+                             tw_entries_c(kTwSpecs[3]) + tw_entries_c(kTwSpecs[4]) + tw_entries_c(kTwSpecs[5]);
+constexpr size_t kTabEntries = kTabC1Col + 2 * NCH;
+
+void tables_fill_host(float2 *host) {
     size_t o = FFT_TW_N;
-    fft_tw_fill(host.data());
+    fft_tw_fill(host);
     for (int i = 1; i < 6; ++i) {
-        off[i] = o;
-        tw_fill(host.data() + o, specs[i]);
-        o += tw_entries(specs[i]);
+        tw_fill(host + o, kTwSpecs[i]);
+        o += tw_entries(kTwSpecs[i]);
     }
     // stage C1 column twiddles e^{-2 pi i c/M} and e^{2 pi i c/L}, c < 16384
-    {
-        const double two_pi = 6.283185307179586476925286766559;
-        for (int c = 0; c < NCH; ++c) {
-            const double am = -two_pi * (double)c / (2.0 * (double)MP);
-            const double al = two_pi * (double)c / (2.0 * (double)P20);
-            host[c1col_off + c] = make_float2((float)cos(am), (float)sin(am));
-            host[c1col_off + NCH + c] = make_float2((float)cos(al), (float)sin(al));
-        }
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int c = 0; c < NCH; ++c) {
+        const double am = -two_pi * (double)c / (2.0 * (double)MP);
+        const double al = two_pi * (double)c / (2.0 * (double)P20);
+        host[kTabC1Col + c] = make_float2((float)cos(am), (float)sin(am));
+        host[kTabC1Col + NCH + c] = make_float2((float)cos(al), (float)sin(al));
     }
-    HIP_TRY(h, hipMalloc(&h->tables, total * sizeof(float2)));
-    HIP_TRY(h, hipMemcpy(h->tables, host.data(), total * sizeof(float2), hipMemcpyHostToDevice));
-    h->tb.fft = h->tables;
-    h->tb.c1col = h->tables + c1col_off;
-    Tw2 *dst[6] = {nullptr, &h->tb.n32k, &h->tb.nmp, &h->tb.nm, &h->tb.nl, &h->tb.np};
+}
+
+// Table pointers into a device copy of the tables_fill_host image.
+Tables tables_bind(const float2 *dev) {
+    Tables tb{};
+    tb.fft = dev;
+    tb.c1col = dev + kTabC1Col;
+    Tw2 *dst[6] = {nullptr, &tb.n32k, &tb.nmp, &tb.nm, &tb.nl, &tb.np};
+    size_t o = FFT_TW_N;
     for (int i = 1; i < 6; ++i) {
-        dst[i]->lo = h->tables + off[i];
-        dst[i]->hi = h->tables + off[i] + ((size_t)1 << specs[i].lo_bits);
-        dst[i]->lo_bits = specs[i].lo_bits;
+        dst[i]->lo = dev + o;
+        dst[i]->hi = dev + o + ((size_t)1 << kTwSpecs[i].lo_bits);
+        dst[i]->lo_bits = kTwSpecs[i].lo_bits;
+        o += tw_entries(kTwSpecs[i]);
     }
+    return tb;
+}
+
+int build_tables(sdbe_handle_t *h) {
+    std::vector<float2> host(kTabEntries);
+    tables_fill_host(host.data());
+    HIP_TRY(h, hipMalloc(&h->tables, kTabEntries * sizeof(float2)));
+    HIP_TRY(h, hipMemcpy(h->tables, host.data(), kTabEntries * sizeof(float2), hipMemcpyHostToDevice));
+    h->tb = tables_bind(h->tables);
     return SDBE_OK;
 }
 
@@ -1285,9 +1307,11 @@ int build_trim_tables(sdbe_handle_t *h) {
 }
 
 // Slot -> column table of the stage C1 output (same rule as c1_column); every
-// column 0..16383 must own exactly one slot.
-int build_slot_table(sdbe_handle_t *h) {
-    std::vector<int> tab(C2_SLOT_TAB, -1), seen(NCH, 0);
+// column 0..16383 must own exactly one slot (false otherwise).  seen: NCH ints
+// of scratch.
+bool slot_table_fill_host(int *tab, int *seen) {
+    for (int i = 0; i < C2_SLOT_TAB; ++i) tab[i] = -1;
+    for (int c = 0; c < NCH; ++c) seen[c] = 0;
     for (int tile = 0; tile < C1_TILES; ++tile) {
         const int a = tile / C1_CLASS_TILES, u = tile - a * C1_CLASS_TILES;
         for (int col = 0; col < 32; ++col) {
@@ -1300,7 +1324,13 @@ int build_slot_table(sdbe_handle_t *h) {
         }
     }
     for (int c = 0; c < NCH; ++c)
-        if (seen[c] != 1) return SDBE_ERR_STATE;
+        if (seen[c] != 1) return false;
+    return true;
+}
+
+int build_slot_table(sdbe_handle_t *h) {
+    std::vector<int> tab(C2_SLOT_TAB), seen(NCH);
+    if (!slot_table_fill_host(tab.data(), seen.data())) return SDBE_ERR_STATE;
     HIP_TRY(h, hipMemcpy(h->slot_col, tab.data(), C2_SLOT_TAB * sizeof(int), hipMemcpyHostToDevice));
     return SDBE_OK;
 }
@@ -1340,10 +1370,10 @@ int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od,
         const BpDesc *bpd = d_bpd + bp0;
         LAUNCH(h, KID_TRIM_C2R, nb, s, kc2r, dim3(D * nb), dim3(1024), bpd, h->V, h->tb);
         if (f32)
-            LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_f, dim3(trim::NSEG * nb), dim3(trim::SEG_THREADS),
+            LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_f, dim3(trim::NSEG * nb), dim3(trim::seg_threads(D)),
                    (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip);
         else
-            LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_c, dim3(trim::NSEG * nb), dim3(trim::SEG_THREADS),
+            LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_c, dim3(trim::NSEG * nb), dim3(trim::seg_threads(D)),
                    (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip);
         LAUNCH(h, KID_STATS, nb, s, k_trim_stats, dim3(nb), dim3(trim::NSEG), (const double *)h->part, P, bp0, h->ms,
                d_stats);
@@ -1370,6 +1400,8 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
                   double *d_stats, hipStream_t s, const uint32_t *pk = nullptr) {
     if (h->trim.frac == 6) return run_trim<39, 32>(h, d_bpd, n_bp, od, f32, d_stats, s);
     if (h->trim.frac == 8) return run_trim<13, 8>(h, d_bpd, n_bp, od, f32, d_stats, s);
+    if (h->trim.frac == 10) return run_trim<65, 64>(h, d_bpd, n_bp, od, f32, d_stats, s);
+    if (h->trim.frac == 11) return run_trim<143, 128>(h, d_bpd, n_bp, od, f32, d_stats, s);
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
@@ -1487,11 +1519,11 @@ sdbe_stream_state_t state_plus(const sdbe_stream_state_t &st, int64_t n_frames) 
 // the int8 rows (k_reorder) or, for the packet-reading stage A, only the fid
 // bitmap and the unique count.
 int unpack_map(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero, hipStream_t s) {
-    LAUNCH(h, KID_FILL, 1, s, k_fill_i32, dim3((MAP_N + 255) / 256), dim3(256), h->map, MAP_N, MAP_NONE);
+    LAUNCH(h, KID_FILL, 1, s, k_fill_i32, dim3((h->map_n + 255) / 256), dim3(256), h->map, h->map_n, MAP_NONE);
     LAUNCH(h, KID_FILL, 0, s, k_fill_i32, dim3(1), dim3(256), h->cnt, CNT_N, 0);
     if (n_pkts > 0)
         LAUNCH(h, KID_MAP, 1, s, k_packet_map, dim3((n_pkts + 255) / 256), dim3(256), (const uint32_t *)d_pkts,
-               n_pkts, (uint32_t)(b_zero & 0xFFFFFFFFull), h->map, h->cnt);
+               n_pkts, (uint32_t)(b_zero & 0xFFFFFFFFull), h->nfr, h->map, h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
 }
@@ -1500,8 +1532,8 @@ int unpack_to_rows(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_
                    uint8_t *d_bitmap, hipStream_t s) {
     int rc = unpack_map(h, d_pkts, n_pkts, b_zero, s);
     if (rc) return rc;
-    LAUNCH(h, KID_REORDER, 1, s, k_reorder, dim3(128, NFR), dim3(256), (const uint32_t *)d_pkts,
-           (const int *)h->map, h->beng, d_bitmap ? d_bitmap : h->bitmap, h->cnt);
+    LAUNCH(h, KID_REORDER, 1, s, k_reorder, dim3(128, h->nfr), dim3(256), (const uint32_t *)d_pkts,
+           (const int *)h->map, h->nfr * 128, h->beng, d_bitmap ? d_bitmap : h->bitmap, h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
 }
@@ -1510,7 +1542,7 @@ int unpack_bitmap(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t
                   uint8_t *d_bitmap, hipStream_t s) {
     int rc = unpack_map(h, d_pkts, n_pkts, b_zero, s);
     if (rc) return rc;
-    LAUNCH(h, KID_BITMAP, 1, s, k_group_bitmap, dim3(NFR), dim3(256), (const int *)h->map,
+    LAUNCH(h, KID_BITMAP, 1, s, k_group_bitmap, dim3(h->nfr), dim3(256), (const int *)h->map,
            d_bitmap ? d_bitmap : h->bitmap, h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
@@ -1631,15 +1663,18 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     h->c1_tpw = cfg.c1_tiles_per_wg ? cfg.c1_tiles_per_wg : C1_TILES_PER_WG;
     h->trim = tg;
     if (tg.D) {
+        // a group is D B-frames of 128 snapshots: 128 blocks of D snapshots
         h->snaps = tg.D;
         h->fpb = tg.E;
+        h->nfr = tg.D;
+        h->map_n = tg.D * 256 * 8;
     }
     h->cap_bp = 2 * SDBE_BLOCKS_PER_GROUP;
     if (cfg.max_spectra_blockpols > h->cap_bp) h->cap_bp = cfg.max_spectra_blockpols;
     {
         const int mb = cfg.max_batch_blockpols ? cfg.max_batch_blockpols : 1024;
         if (mb > h->cap_bp) h->cap_bp = mb;
-        if (tg.D && 2 * (NSNAP_GROUP / tg.D) > h->cap_bp) h->cap_bp = 2 * (NSNAP_GROUP / tg.D);
+        if (2 * SDBE_BLOCKS_PER_GROUP > h->cap_bp) h->cap_bp = 2 * SDBE_BLOCKS_PER_GROUP;
     }
     h->st.psn = 1;
     int rc = SDBE_OK;
@@ -1661,16 +1696,20 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
             goto fail;                                 \
         }                                              \
     } while (0)
-    ALLOC(h->map, MAP_N * sizeof(int));
-    ALLOC(h->beng, 2 * (size_t)NSNAP_GROUP * NCH * 2);
-    ALLOC(h->bitmap, NFR * 256);
+    ALLOC(h->map, (size_t)h->map_n * sizeof(int));
+    ALLOC(h->beng, 2 * (size_t)h->nfr * 128 * NCH * 2);
+    ALLOC(h->bitmap, (size_t)h->nfr * 256);
     ALLOC(h->cnt, CNT_N * sizeof(int));
     ALLOC(h->latch, 2 * SDBE_LATCH_SLOTS * sizeof(float));
     for (int i = 0; i < sdbe_handle::RING; ++i) ALLOC(h->d_bpd[i], (size_t)h->cap_bp * sizeof(BpDesc));
     ALLOC(h->g_d, (size_t)h->cap_bp * sizeof(BpDesc));
     ALLOC(h->lead, 2 * OUT_FRAME_PAYLOAD);
-    ALLOC(h->V, C * NBLK * NCH * sizeof(float2));
-    ALLOC(h->Zf, C * MP * sizeof(float2));
+    // V: stage A's rows (64/39) or the block's concatenated C2R series
+    // (band-trimmed, D snapshots of 32768 floats); Zf: stage B's packed
+    // spectrum (64/39) or the band-trimmed samples (128 segments of P floats)
+    ALLOC(h->V, C * (size_t)(h->snaps > NBLK ? h->snaps : NBLK) * NCH * sizeof(float2));
+    ALLOC(h->Zf, C * (tg.D && (size_t)trim::NSEG * tg.P * sizeof(float) > MP * sizeof(float2)
+                          ? (size_t)trim::NSEG * tg.P * sizeof(float) : MP * sizeof(float2)));
     ALLOC(h->Bc, C * KHI * BC_ROW * sizeof(float2));
     ALLOC(h->slot_col, C2_SLOT_TAB * sizeof(int));
     ALLOC(h->part, C * C1_TILES * 2 * sizeof(double));
@@ -1763,12 +1802,12 @@ int sdbe_stream_state_from_template(const void *tmpl, int32_t n_words_per_chan, 
 int sdbe_unpack_group_int8(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero,
                            int8_t *d_beng0, int8_t *d_beng1, uint8_t *d_bitmap, void *stream) {
     if (!h || !d_beng0 || !d_beng1 || (n_pkts > 0 && !d_pkts)) return SDBE_ERR_ARG;
-    if (n_pkts < 0 || n_pkts > MAP_N) return SDBE_ERR_CAPACITY;
+    if (n_pkts < 0 || n_pkts > h->map_n) return SDBE_ERR_CAPACITY;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
     int rc = unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;
-    const size_t pol_bytes = (size_t)NSNAP_GROUP * NCH * 2;
+    const size_t pol_bytes = (size_t)h->nfr * 128 * NCH * 2;
     HIP_TRY(h, hipMemcpyAsync(d_beng0, h->beng, pol_bytes, hipMemcpyDeviceToDevice, s));
     HIP_TRY(h, hipMemcpyAsync(d_beng1, h->beng + pol_bytes, pol_bytes, hipMemcpyDeviceToDevice, s));
     h->n_packets_last = n_pkts;
@@ -1780,9 +1819,10 @@ int sdbe_unpack_group_int8(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
 int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero, void *d_out,
                        int64_t out_pol_stride_bytes, uint8_t *d_bitmap, void *stream) {
     if (!h || !d_out || (n_pkts > 0 && !d_pkts)) return SDBE_ERR_ARG;
-    if (n_pkts < 0 || n_pkts > MAP_N) return SDBE_ERR_CAPACITY;
+    if (n_pkts < 0 || n_pkts > h->map_n) return SDBE_ERR_CAPACITY;
     OutDesc od = out_desc(h);
-    const int blocks = NSNAP_GROUP / h->snaps;           // per polarisation
+    const int nsnap = h->nfr * 128;                      // snapshots per polarisation
+    const int blocks = nsnap / h->snaps;                 // per polarisation (128)
     if (out_pol_stride_bytes < (int64_t)blocks * h->fpb * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
@@ -1799,7 +1839,7 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     const sdbe_config_t &c = h->cfg;
     for (int bp = 0; bp < n_bp; ++bp) {
         const int blk = bp >> 1, pol = bp & 1;
-        const int8_t *rows = h->beng + ((size_t)pol * NSNAP_GROUP + (size_t)h->snaps * blk) * NCH * 2;
+        const int8_t *rows = h->beng + ((size_t)pol * nsnap + (size_t)h->snaps * blk) * NCH * 2;
         uint8_t *out = (uint8_t *)d_out + (int64_t)pol * out_pol_stride_bytes +
                        (int64_t)blk * h->fpb * od.frame_bytes;
         set_bp(hb[bp], rows, out, state_plus(h->st, (int64_t)blk * h->fpb), pol, 0, c.thresh_policy, 0,
@@ -2058,7 +2098,7 @@ int sdbe_get_stats(sdbe_handle_t *h, sdbe_stats_t *o) {
     o->n_out_of_range = h->h_cnt[CNT_RANGE];
     o->n_unique = h->h_cnt[CNT_UNIQUE];
     o->n_duplicate = h->h_cnt[CNT_ACCEPTED] - h->h_cnt[CNT_UNIQUE];
-    o->n_missing = h->have_group ? MAP_N - h->h_cnt[CNT_UNIQUE] : 0;
+    o->n_missing = h->have_group ? h->map_n - h->h_cnt[CNT_UNIQUE] : 0;
     o->n_blockpols = h->n_bp_last;
     o->n_frames_per_pol = h->n_frames_last;
     o->latched_threshold = h->h_latch[0];
@@ -2119,3 +2159,4 @@ int sdbe_profile_read(sdbe_handle_t *h, sdbe_kernel_time_t *out, int32_t max_rec
 }
 
 }  // extern "C"
+

@@ -1,6 +1,6 @@
-// Band-trimmed rational resampler of the aphids GPU thread at a reduced SWARM
-// rate (software/aphids/vdif_inout_gpu_thread.cu:589-709, plans :269-333;
-// presets sample_rates.h:21-40): per block of D snapshots
+// Band-trimmed rational resampler of the aphids GPU thread at a SWARM rate
+// preset (software/aphids/vdif_inout_gpu_thread.cu:589-709, plans :269-333;
+// presets sample_rates.h:21-63): per block of D snapshots
 //   SwarmC2R  : inverse real DFT of 32768 points per snapshot (k_trim_c2r),
 //   SwarmR2C  : real DFTs of R = 256 D samples over the concatenation, 128 segments,
 //   R2dbeC2R  : inverse real DFT of P = 256 E points from bins skip .. skip + P/2
@@ -15,8 +15,11 @@
 //   W[j] = (Y[j] + conj Y[P/2-j]) + i e^{2 pi i j/P} (Y[j] - conj Y[P/2-j]),
 // Y[j] = X[skip + j] with the imaginary parts of Y[0] and Y[P/2] dropped (numpy
 // irfft semantics, as the rest of the stage).  The R/2- and P/2-point FFTs are
-// mixed-radix Stockham passes through two LDS buffers (4992 = 16 8 13 3,
-// 1664 = 16 8 13, 4096 = 16 16 16, 1024 = 16 8 8).
+// mixed-radix Stockham passes in place in one LDS buffer:
+//   6/11  (D 39, E 32):   4992 = 16 8 13 3,   4096 = 16 16 16
+//   8/11  (D 13, E 8):    1664 = 16 8 13,     1024 = 16 8 8
+//   10/11 (D 65, E 64):   8320 = 16 8 5 13,   8192 = 16 16 16 2
+//   11/11 (D 143, E 128): 18304 = 16 8 11 13, 16384 = 16 16 16 4
 //
 // Units: the unnormalised chain is numpy's irfft / rfft / irfft times
 // 32768 P; k_trim_seg removes that exact power of two, so samples, statistics
@@ -26,7 +29,10 @@
 namespace trim {
 
 constexpr int NSEG = 128;             // segments per block (D 32768 / (256 D))
-constexpr int SEG_THREADS = 256;
+// Threads of a segment workgroup: 256 while the R/2-point buffer is small
+// (6/11, 8/11: up to 39 KB, four workgroups per CU), 1024 for the 66 KB
+// (10/11) and 146 KB (11/11) buffers, where one or two workgroups fill a CU.
+__host__ __device__ constexpr int seg_threads(int D) { return 128 * D > 5000 ? 1024 : 256; }
 
 // Host: geometry of a preset.  Returns false for an unsupported preset or a
 // skip that leaves fewer than P/2 + 1 bins.
@@ -41,6 +47,12 @@ inline bool geometry(int rate_frac, int skip_mhz, Geometry &g) {
     } else if (rate_frac == 8) {
         D = 13;
         E = 8;
+    } else if (rate_frac == 10) {
+        D = 65;
+        E = 64;
+    } else if (rate_frac == 11) {
+        D = 143;
+        E = 128;
     } else {
         return false;
     }
@@ -61,6 +73,56 @@ inline bool geometry(int rate_frac, int skip_mhz, Geometry &g) {
 // 16 groups (N is a multiple of 16 for every size here).
 __device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
 
+// DFT of odd prime length P by conjugate pairing, in place, sign S:
+// X[d] = x0 + A_d + S i B_d and X[P-d] = x0 + A_d - S i B_d with
+// A_d = sum_j (x[j] + x[P-j]) cos(2 pi j d/P), B_d = sum_j (x[j] - x[P-j]) sin(2 pi j d/P).
+// c[m], s[m]: cos and sin of 2 pi m/P for m <= (P-1)/2.
+template <int P, int S>
+__device__ __forceinline__ void dft_pairs(float2 *x, const float *c, const float *s) {
+    constexpr int H = (P - 1) / 2;
+    float2 sp[H + 1], dm[H + 1];
+    float2 tot = x[0];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        sp[j] = cadd(x[j], x[P - j]);
+        dm[j] = csub(x[j], x[P - j]);
+        tot = cadd(tot, sp[j]);
+    }
+    const float2 x0 = x[0];
+    x[0] = tot;
+#pragma unroll
+    for (int d = 1; d <= H; ++d) {
+        float2 A = x0, B = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const int m = (j * d) % P;
+            const float cm = m <= H ? c[m] : c[P - m];
+            const float sm = m <= H ? s[m] : -s[P - m];
+            A.x = fmaf(sp[j].x, cm, A.x);
+            A.y = fmaf(sp[j].y, cm, A.y);
+            B.x = fmaf(dm[j].x, sm, B.x);
+            B.y = fmaf(dm[j].y, sm, B.y);
+        }
+        const float2 iB = mul_si<S>(B);
+        x[d] = cadd(A, iB);
+        x[P - d] = csub(A, iB);
+    }
+}
+template <int S>
+__device__ __forceinline__ void dft5(float2 *x) {
+    const float c[3] = {1.0f, 3.0901699437e-01f, -8.0901699437e-01f};
+    const float s[3] = {0.0f, 9.5105651630e-01f, 5.8778525229e-01f};
+    dft_pairs<5, S>(x, c, s);
+}
+template <int S>
+__device__ __forceinline__ void dft11(float2 *x) {
+    const float c[6] = {1.0f, 8.4125353283e-01f, 4.1541501300e-01f, -1.4231483827e-01f, -6.5486073395e-01f,
+                        -9.5949297361e-01f};
+    const float s[6] = {0.0f, 5.4064081746e-01f, 9.0963199535e-01f, 9.8982144188e-01f, 7.5574957435e-01f,
+                        2.8173255684e-01f};
+    dft_pairs<11, S>(x, c, s);
+}
+
 // DFT of radix R in place, natural order in and out, sign S.
 template <int R, int S>
 __device__ __forceinline__ void dft_r(float2 *x) {
@@ -70,8 +132,12 @@ __device__ __forceinline__ void dft_r(float2 *x) {
         dft3<S>(x[0], x[1], x[2]);
     } else if constexpr (R == 4) {
         dft4<S>(x[0], x[1], x[2], x[3]);
+    } else if constexpr (R == 5) {
+        dft5<S>(x);
     } else if constexpr (R == 8) {
         dft8<S>(x);
+    } else if constexpr (R == 11) {
+        dft11<S>(x);
     } else if constexpr (R == 13) {
         dft13<S>(x);
     } else {
@@ -90,16 +156,16 @@ __device__ __forceinline__ void dft_r(float2 *x) {
 // Loads of a partial last iteration read butterfly N/R - 1; only its writes are
 // guarded.  In place: all reads of the pass happen before the barrier that
 // precedes the first write.
-template <int N, int R, int NS, int S>
+template <int N, int R, int NS, int S, int T>
 __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tab) {
     const float2 *in = buf;
     float2 *out = buf;
-    constexpr int NB = N / R, IT = (NB + SEG_THREADS - 1) / SEG_THREADS, STRIDE = N / (NS * R);
+    constexpr int NB = N / R, IT = (NB + T - 1) / T, STRIDE = N / (NS * R);
     float2 x[IT][R];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int b0 = threadIdx.x + it * SEG_THREADS;
-        const int b = (NB % SEG_THREADS == 0 || it < IT - 1) ? b0 : (b0 < NB ? b0 : NB - 1);
+        const int b0 = threadIdx.x + it * T;
+        const int b = (NB % T == 0 || it < IT - 1) ? b0 : (b0 < NB ? b0 : NB - 1);
 #pragma unroll
         for (int i = 0; i < R; ++i) x[it][i] = in[swz16(b + NB * i)];
         if constexpr (NS > 1) {
@@ -122,9 +188,9 @@ __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tab
     __syncthreads();   // every read of the pass is done
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int b = threadIdx.x + it * SEG_THREADS;
+        const int b = threadIdx.x + it * T;
         dft_r<R, S>(x[it]);
-        if (NB % SEG_THREADS == 0 || it < IT - 1 || b < NB) {
+        if (NB % T == 0 || it < IT - 1 || b < NB) {
             const int base = (b / NS) * NS * R + (b % NS);
 #pragma unroll
             for (int k = 0; k < R; ++k) out[swz16(base + NS * k)] = x[it][k];
@@ -133,27 +199,47 @@ __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tab
     __syncthreads();
 }
 
-// Radix sequences (first pass Ns = 1, no twiddles), in place.
-template <int N, int S>
+// Radix sequences (first pass Ns = 1, no twiddles), in place, T threads.
+template <int N, int S, int T>
 __device__ __forceinline__ void fft(float2 *a, const float2 *__restrict__ tab) {
     if constexpr (N == 4992) {
-        pass<N, 16, 1, S>(a, tab);
-        pass<N, 8, 16, S>(a, tab);
-        pass<N, 13, 128, S>(a, tab);
-        pass<N, 3, 1664, S>(a, tab);
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 8, 16, S, T>(a, tab);
+        pass<N, 13, 128, S, T>(a, tab);
+        pass<N, 3, 1664, S, T>(a, tab);
     } else if constexpr (N == 1664) {
-        pass<N, 16, 1, S>(a, tab);
-        pass<N, 8, 16, S>(a, tab);
-        pass<N, 13, 128, S>(a, tab);
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 8, 16, S, T>(a, tab);
+        pass<N, 13, 128, S, T>(a, tab);
+    } else if constexpr (N == 8320) {
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 8, 16, S, T>(a, tab);
+        pass<N, 5, 128, S, T>(a, tab);
+        pass<N, 13, 640, S, T>(a, tab);
+    } else if constexpr (N == 18304) {
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 8, 16, S, T>(a, tab);
+        pass<N, 11, 128, S, T>(a, tab);
+        pass<N, 13, 1408, S, T>(a, tab);
     } else if constexpr (N == 4096) {
-        pass<N, 16, 1, S>(a, tab);
-        pass<N, 16, 16, S>(a, tab);
-        pass<N, 16, 256, S>(a, tab);
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 16, 16, S, T>(a, tab);
+        pass<N, 16, 256, S, T>(a, tab);
+    } else if constexpr (N == 8192) {
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 16, 16, S, T>(a, tab);
+        pass<N, 16, 256, S, T>(a, tab);
+        pass<N, 2, 4096, S, T>(a, tab);
+    } else if constexpr (N == 16384) {
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 16, 16, S, T>(a, tab);
+        pass<N, 16, 256, S, T>(a, tab);
+        pass<N, 4, 4096, S, T>(a, tab);
     } else {
         static_assert(N == 1024, "size");
-        pass<N, 16, 1, S>(a, tab);
-        pass<N, 8, 16, S>(a, tab);
-        pass<N, 8, 128, S>(a, tab);
+        pass<N, 16, 1, S, T>(a, tab);
+        pass<N, 8, 16, S, T>(a, tab);
+        pass<N, 8, 128, S, T>(a, tab);
     }
 }
 
@@ -200,24 +286,26 @@ k_trim_c2r(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb) {
     for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
 }
 
-// SwarmR2C + R2dbeC2R of segment g of block bpl (workgroup = segment, 256
-// threads).  Output: P samples (numpy units) at out[bpl] + g P, where out is
-// bpd[bpl].out (float samples) when F32, else stage + bpl * 128 P; FP64 sums of
-// x and x^2 of the segment at part[(bpl * 128 + g) * 2].
+// SwarmR2C + R2dbeC2R of segment g of block bpl (workgroup = segment,
+// trim::seg_threads(D) threads).  Output: P samples (numpy units) at
+// out[bpl] + g P, where out is bpd[bpl].out (float samples) when F32, else
+// stage + bpl * 128 P; FP64 sums of x and x^2 of the segment at
+// part[(bpl * 128 + g) * 2].
 template <int D, int E, bool F32>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(trim::seg_threads(D))
 k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *__restrict__ stage,
            double *__restrict__ part, trim::Tabs tt, int skip) {
     constexpr int R = 256 * D, P = 256 * E, NR = R / 2, NP = P / 2;
     constexpr int NBUF = NR > NP ? NR : NP;
-    // one in-place buffer: 39 KB at 6/11, four workgroups per CU
+    constexpr int NT = trim::seg_threads(D);
+    // one in-place buffer: 39 KB at 6/11 (four workgroups per CU), 146 KB at 11/11
     __shared__ float2 buf[NBUF];
-    __shared__ double red[2][trim::SEG_THREADS / 64];
+    __shared__ double red[2][NT / 64];
     const int bpl = blockIdx.x / trim::NSEG, g = blockIdx.x - bpl * trim::NSEG, t = threadIdx.x;
     const float2 *z = reinterpret_cast<const float2 *>(T + (size_t)bpl * D * 2 * NCH + (size_t)g * R);
-    for (int m = t; m < NR; m += trim::SEG_THREADS) buf[trim::swz16(m)] = z[m];
+    for (int m = t; m < NR; m += NT) buf[trim::swz16(m)] = z[m];
     __syncthreads();
-    trim::fft<NR, -1>(buf, tt.zr);
+    trim::fft<NR, -1, NT>(buf, tt.zr);
     // bins skip .. skip + NP of the R2C, then the C2R pre-processing; the NP / 256
     // values of a thread stay in registers until every read of Z is done
     auto X = [&](int k) {
@@ -225,12 +313,12 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
         const float2 e = cscale(cadd(a, b), 0.5f), d = cscale(csub(a, b), 0.5f);
         return cadd(e, mul_si<-1>(cmul(tt.xr[k], d)));
     };
-    constexpr int JT = NP / trim::SEG_THREADS;
-    static_assert(NP % trim::SEG_THREADS == 0, "C2R size");
+    constexpr int JT = NP / NT;
+    static_assert(NP % NT == 0, "C2R size");
     float2 wv[JT];
 #pragma unroll
     for (int q = 0; q < JT; ++q) {
-        const int j = t + trim::SEG_THREADS * q;
+        const int j = t + NT * q;
         float2 y = X(skip + j), ym = X(skip + NP - j);
         if (j == 0) {
             y.y = 0.f;         // Y[0] and Y[P/2] enter with their real parts only
@@ -241,9 +329,9 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < JT; ++q) buf[trim::swz16(t + trim::SEG_THREADS * q)] = wv[q];
+    for (int q = 0; q < JT; ++q) buf[trim::swz16(t + NT * q)] = wv[q];
     __syncthreads();
-    trim::fft<NP, +1>(buf, tt.zp);
+    trim::fft<NP, +1, NT>(buf, tt.zp);
     const float2 *w = buf;
     // samples (2m, 2m+1) = (Re, Im) w[m] / (32768 P)
     const float sc = 1.0f / (32768.0f * (float)P);
@@ -251,7 +339,7 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
                                              (size_t)g * P);
     double s1 = 0.0, s2 = 0.0;
 #pragma unroll 4
-    for (int m = t; m < NP; m += trim::SEG_THREADS) {
+    for (int m = t; m < NP; m += NT) {
         const float2 x = cscale(w[trim::swz16(m)], sc);
         dst[m] = x;
         s1 += (double)x.x + (double)x.y;
@@ -269,7 +357,7 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
     __syncthreads();
     if (t == 0) {
         double a = 0.0, b = 0.0;
-        for (int i = 0; i < trim::SEG_THREADS / 64; ++i) {
+        for (int i = 0; i < NT / 64; ++i) {
             a += red[0][i];
             b += red[1][i];
         }

@@ -13,13 +13,27 @@
  * reference threshold policy and payload-only output (the layout of
  * vdif_out_data_group_t), so the reference output thread consumes it as is.
  *
+ * Resampler, chosen at compile time as the reference chooses its rate
+ * (sample_rates.h:12-63):
+ *   SDBE_PLUGIN_TRIMMED = 0 (default): the 39 -> 64 resampler of the
+ *     prototypes (resample.py fft_batched), 6/11 rate only, 4096 Msps out;
+ *   SDBE_PLUGIN_TRIMMED = 1: the reference thread's band-trimmed resampler at
+ *     SWARM_RATE_FRAC (6, 8, 10 or 11; aphids_resampler_init,
+ *     vdif_inout_gpu_thread.cu:184-347), trimming the MHz read once at start
+ *     from the status key TRIMDC (the reference reads the redis key
+ *     trim_from_dc, :770-781; 0 when the key is absent).
+ * N_32bit_words_per_chan is (2 * 16384 * 128 * EXPANSION) / 16 words, as the
+ * reference sets it (:1012), with EXPANSION = 64 for the 39 -> 64 resampler.
+ *
  * Differences from the reference, all deliberate:
  *   - one device output buffer per output slot (the reference reuses a single
  *     buffer for every slot, which the output thread may still be copying);
  *   - HIP errors are reported through hashpipe_error and put the thread into
  *     its error state (the reference ignores most CUDA errors);
- *   - no redis context: status and the trim_from_dc key are not read (this
- *     stage implements the 64/39 resampling, which has no trim).
+ *   - no redis context: the trim comes from the hashpipe status buffer;
+ *   - band-trimmed frames follow each other per block (E per block and
+ *     polarisation), where the reference writes E of every
+ *     VDIF_OUT_PKTS_PER_BLOCK / 128 payload slots (the same at 10/11 and 11/11).
  */
 #include <stdio.h>
 #include <string.h>
@@ -35,17 +49,37 @@
 #define STATE_PROC 1
 #define MAX_GPUS 8
 
+#ifndef SDBE_PLUGIN_TRIMMED
+#define SDBE_PLUGIN_TRIMMED 0
+#endif
+#if SDBE_PLUGIN_TRIMMED
+#define PLUGIN_RATE SWARM_RATE_FRAC
+#define PLUGIN_EXPANSION EXPANSION_FACTOR
+#else
+#if SWARM_RATE_FRAC != 6
+#error "the 39 -> 64 resampler takes 39-snapshot blocks of the 6/11 rate: build with -DSDBE_PLUGIN_TRIMMED=1"
+#endif
+#define PLUGIN_RATE SDBE_RATE_64_39
+#define PLUGIN_EXPANSION 64
+#endif
+/* 2-bit words per polarisation and group: 128 blocks of 32768 E samples */
+#define PLUGIN_WORDS_PER_CHAN ((2 * 16384 * 128 * PLUGIN_EXPANSION) / (32 / 2))
+_Static_assert((PLUGIN_WORDS_PER_CHAN / (VDIF_OUT_PKT_DATA_SIZE / 4)) <= VDIF_OUT_PKTS_PER_BLOCK,
+               "a group's frames fit the output slot");
+
 typedef struct {
     sdbe_handle_t *h;
     vdif_out_data_group_t *out[VDIF_OUT_BUFFER_SIZE];
 } gpu_slot_t;
 
-static int ensure_gpu(gpu_slot_t *g, int dev) {
+static int ensure_gpu(gpu_slot_t *g, int dev, int trim_mhz) {
     if (g->h) return 0;
     sdbe_config_t cfg;
     sdbe_default_config(&cfg);
     cfg.thresh_policy = SDBE_THRESH_REFERENCE;
     cfg.out_mode = SDBE_OUT_PAYLOAD;
+    cfg.rate_frac = PLUGIN_RATE;
+    cfg.skip_mhz = SDBE_PLUGIN_TRIMMED ? trim_mhz : 0;
     int rc = sdbe_create(&cfg, dev, &g->h);
     if (rc != SDBE_OK) {
         hashpipe_error("vdif_inout_gpu_thread", "sdbe_create(device %d): %s", dev, sdbe_error_string(rc));
@@ -72,6 +106,17 @@ static void release_gpus(gpu_slot_t *gs) {
     }
 }
 
+/* MHz trimmed from DC: status key TRIMDC, read once (0 when absent). */
+static int read_trim_mhz(hashpipe_thread_args_t *args) {
+    int mhz = 0;
+    hashpipe_status_t st = args->st;
+    if (!st.buf) return 0;
+    hashpipe_status_lock_safe(&st);
+    if (!hgeti4(st.buf, "TRIMDC", &mhz)) mhz = 0;
+    hashpipe_status_unlock_safe(&st);
+    return mhz;
+}
+
 static void *run_method(hashpipe_thread_args_t *args) {
     vdif_in_databuf_t *db_in = (vdif_in_databuf_t *)args->ibuf;
     vdif_out_databuf_t *db_out = (vdif_out_databuf_t *)args->obuf;
@@ -79,6 +124,7 @@ static void *run_method(hashpipe_thread_args_t *args) {
     memset(gpus, 0, sizeof(gpus));
     int index_in = 0, index_out = 0, rv;
     int state = STATE_INIT;
+    const int trim_mhz = read_trim_mhz(args);
 
     while (run_threads()) {
         if (state == STATE_ERROR) {
@@ -109,7 +155,7 @@ static void *run_method(hashpipe_thread_args_t *args) {
 
         beng_group_completion_t bgc = db_in->bgc[index_in];
         const int dev = bgc.gpu_id;
-        if (dev < 0 || dev >= MAX_GPUS || hipSetDevice(dev) != hipSuccess || ensure_gpu(&gpus[dev], dev) != 0) {
+        if (dev < 0 || dev >= MAX_GPUS || hipSetDevice(dev) != hipSuccess || ensure_gpu(&gpus[dev], dev, trim_mhz) != 0) {
             hashpipe_error(__func__, "cannot use GPU %d", dev);
             state = STATE_ERROR;
             continue;
@@ -131,7 +177,7 @@ static void *run_method(hashpipe_thread_args_t *args) {
 
         quantized_storage_t *qs = &db_out->blocks[index_out];
         qs->bit_depth = 2;
-        qs->N_32bit_words_per_chan = SDBE_OUT_FRAMES_PER_GROUP * (VDIF_OUT_PKT_DATA_SIZE / 4);
+        qs->N_32bit_words_per_chan = PLUGIN_WORDS_PER_CHAN;
         qs->gpu_id = dev;
         qs->vdg_buf_gpu = out;
         memcpy(&qs->vdif_header_template, &bgc.vdif_header_template, sizeof(vdif_in_header_t));

@@ -50,11 +50,20 @@ class BengFrameCompletion(C.Structure):
     _fields_ = [("b", C.c_uint64), ("beng_frame_vdif_packet_count", C.c_uint16), ("cc", C.c_uint8 * 256)]
 
 
-class BengGroupCompletion(C.Structure):
-    _fields_ = [("beng_group_vdif_packet_count", C.c_int32), ("bgv_buf_cpu", C.c_void_p),
-                ("bgv_buf_gpu", C.c_void_p), ("gpu_id", C.c_int), ("memcpy_stream", C.c_void_p),
-                ("bfc", BengFrameCompletion * BENG_FRAMES_PER_GROUP),
-                ("vdif_header_template", VdifInHeader)]
+def group_completion_type(frames):
+    """beng_group_completion_t of a build whose BENG_FRAMES_PER_GROUP is
+    `frames` (DECIMATION_FACTOR of its SWARM rate preset)."""
+    class _Bgc(C.Structure):
+        _fields_ = [("beng_group_vdif_packet_count", C.c_int32), ("bgv_buf_cpu", C.c_void_p),
+                    ("bgv_buf_gpu", C.c_void_p), ("gpu_id", C.c_int), ("memcpy_stream", C.c_void_p),
+                    ("bfc", BengFrameCompletion * frames),
+                    ("vdif_header_template", VdifInHeader)]
+    _Bgc.__name__ = "BengGroupCompletion%d" % frames
+    assert C.sizeof(_Bgc) == 72 + 272 * frames
+    return _Bgc
+
+
+BengGroupCompletion = group_completion_type(BENG_FRAMES_PER_GROUP)
 
 
 class HashpipeDatabuf(C.Structure):
@@ -85,16 +94,29 @@ assert C.sizeof(VdifInHeader) == 32 and C.sizeof(VdifInPacket) == 1056
 assert C.sizeof(BengFrameCompletion) == 272 and C.sizeof(BengGroupCompletion) == 10680
 assert C.sizeof(QuantizedStorage) == 72
 
-_lib = None
+# Plugin builds (sdbe_amd/build.py PLUGIN_VARIANTS): variant name ->
+# (library, SWARM_RATE_FRAC, band-trimmed resampler, B-frames per group,
+# output slots per polarisation VDIF_OUT_PKTS_PER_BLOCK)
+VARIANTS = {
+    None: (LIB_PATH, 6, False, 39, 8192),
+    "trim6": (os.path.join(_HERE, "libsdbe_hashpipe_trim6.so"), 6, True, 39, 8192),
+    "trim8": (os.path.join(_HERE, "libsdbe_hashpipe_trim8.so"), 8, True, 13, 2048),
+    "trim10": (os.path.join(_HERE, "libsdbe_hashpipe_trim10.so"), 10, True, 65, 8192),
+    "trim11": (os.path.join(_HERE, "libsdbe_hashpipe_trim11.so"), 11, True, 143, 16384),
+}
+
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError("libsdbe_hashpipe.so is not built (run `python -m sdbe_amd.build`)")
-    lib = C.CDLL(LIB_PATH)
+def load(variant=None):
+    """The plugin library of a build variant (None: the default 6/11 build of
+    the 39 -> 64 resampler)."""
+    if variant in _libs:
+        return _libs[variant]
+    path = VARIANTS[variant][0]
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built (run `python -m sdbe_amd.build`)" % os.path.basename(path))
+    lib = C.CDLL(path)
     P = C.POINTER
     sigs = {
         "find_hashpipe_thread": (P(ThreadDesc), [C.c_char_p]),
@@ -127,10 +149,22 @@ def load():
         "hpc_set_out_free": (C.c_int, [C.c_int]),
         "hpc_wait_out_filled": (C.c_int, [C.c_int, C.c_int]),
         "hpc_stop": (C.c_int, []),
+        "hpc_pipeline_start": (C.c_int, [C.POINTER(C.c_char_p), C.c_int]),
+        "hpc_thread_done": (C.c_int, [C.c_int]),
+        "hpc_status_puti4": (C.c_int, [C.c_char_p, C.c_int]),
+        "hpc_status_puts": (C.c_int, [C.c_char_p, C.c_char_p]),
+        "hpc_status_geti4": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
+        "hpc_status_clear": (C.c_int, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
+    _libs[variant] = lib
     return lib
+
+
+def pipeline_start(lib, names):
+    """Start the named plugins as a hashpipe pipeline (hpc_pipeline_start)."""
+    arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    return lib.hpc_pipeline_start(arr, len(names))

@@ -55,12 +55,15 @@ class SdbeStage:
         self.cfg = cfg
         # geometry of a block-pol: snapshots in, samples and frames out
         # (include/sdbe_stage.h, SDBE_RATE_*)
+        # a group is 128 blocks: 39 B-frames at 64/39, D B-frames band-trimmed
         if rate_frac == L.RATE_64_39:
             self.snaps, self.samples, self.frames = 39, L.SAMPLES_PER_BLOCK, L.OUT_FRAMES_PER_BLOCK
+            self.group_frames = L.FRAMES_PER_GROUP
         else:
-            d, e = {L.RATE_6_11: (39, 32), L.RATE_8_11: (13, 8)}.get(rate_frac, (0, 0))
+            d, e = L.TRIM_PRESETS.get(rate_frac, (0, 0))
             self.snaps, self.samples, self.frames = d, 128 * 256 * e, e
-        self.frames_per_group = (L.SNAPSHOTS_PER_GROUP // self.snaps) * self.frames if self.snaps else 0
+            self.group_frames = d
+        self.frames_per_group = L.BLOCKS_PER_GROUP * self.frames
         self.device = device
         self.out_mode = out_mode
         self.frame_bytes = L.OUT_PAYLOAD_BYTES + (L.OUT_HEADER_BYTES if out_mode == L.OUT_FRAMES else 0)
@@ -120,7 +123,8 @@ class SdbeStage:
     # -------------------------------------------------------------- buffers
     def group_out(self):
         """Output buffer of one group, [2 pols][frames per group][frame bytes] uint8
-        (8192 frames for the 64/39 resampler, 4096 for 6/11, 3072 for 8/11)."""
+        (128 blocks: 8192 frames for the 64/39 resampler, 128 E band-trimmed:
+        4096 at 6/11, 1024 at 8/11, 8192 at 10/11, 16384 at 11/11)."""
         return torch.empty((2, self.frames_per_group, self.frame_bytes), dtype=torch.uint8,
                            device="cuda:%d" % self.device)
 
@@ -143,7 +147,7 @@ class SdbeStage:
 
     def unpack_int8(self, d_pkts, n_pkts, b_zero, stream=None):
         dev = "cuda:%d" % self.device
-        b0 = torch.empty((39, 128, L.CHANNELS, 2), dtype=torch.int8, device=dev)
+        b0 = torch.empty((self.group_frames, 128, L.CHANNELS, 2), dtype=torch.int8, device=dev)
         b1 = torch.empty_like(b0)
         bm = torch.empty((39, 256), dtype=torch.uint8, device=dev)
         assert d_pkts.numel() >= n_pkts * L.PKT_BYTES

@@ -144,21 +144,87 @@ def shuffle_and_drop(pkts, rng, drop_frac=0.001):
 
 
 def make_group_packets(seed, n_groups=2, drop_frac=0.001, shuffle=True, tones=None,
-                       b_first=1000):
-    """Config-2 style input: ``n_groups`` groups of 39 B-frames, both pols.
-    Returns a list of (packets, b_zero) per group and the per-pol codes."""
+                       b_first=1000, frames=39):
+    """Config-2 style input: ``n_groups`` groups of ``frames`` B-frames (39 at
+    the 6/11 rate; BENG_FRAMES_PER_GROUP = DECIMATION_FACTOR of the rate,
+    software/aphids/sample_rates.h:67), both pols.  Returns a list of
+    (packets, b_zero) per group and the per-pol codes."""
     rng = rng_for(seed)
-    n_snap = 39 * N_SNAP_PER_FRAME * n_groups
+    n_snap = frames * N_SNAP_PER_FRAME * n_groups
     kw = {} if tones is None else {"tones": tones}
     c0 = gaussian_codes(rng, n_snap, **kw)
     c1 = gaussian_codes(rng, n_snap, **kw)
     out = []
-    per = 39 * N_SNAP_PER_FRAME
+    per = frames * N_SNAP_PER_FRAME
     for g in range(n_groups):
-        b0 = b_first + 39 * g
+        b0 = b_first + frames * g
         pk = packetize(c0[g * per:(g + 1) * per], c1[g * per:(g + 1) * per], b0,
-                       df0=g * 39 * 2048)
+                       df0=g * frames * 2048)
         if shuffle:
             pk, _ = shuffle_and_drop(pk, rng, drop_frac)
         out.append((pk, b0))
     return out, (c0, c1)
+
+
+SWARM_CLK_PER_SEC = 286_000_000     # B-engine clock ticks per second (beng_reform.c)
+SWARM_BFRAME_CLK = 262_144          # clock ticks per B-frame
+
+
+def to_8pac_new_format(pkts, b_ref, t_ref):
+    """Old-format packets -> the SWARM wire format the APHIDS input thread
+    receives: new-format headers carrying the 53-bit B-engine timestamp (clock
+    in the low 29 bits, seconds above; word 4 = bits 21..52, word 5 =
+    c | f << 8 | bits 0..20 << 11; software/aphids/beng_reform.c:16-22,137-186)
+    and 8 packets per 8pac record (packet 1, then 7 x (8-byte VTP + packet);
+    the record's df / psn / df_len apply to all 8, vdif_8pac.c:5-56).
+    Packet p's timestamp is t_ref = (sec, clk) plus (B-count - b_ref)
+    B-frames.  Returns uint8 records [n / 8][8504]; n must be a multiple of 8."""
+    pkts = np.ascontiguousarray(pkts, dtype=np.uint32)
+    n = pkts.shape[0]
+    assert n % 8 == 0
+    w = pkts.copy()
+    w4, w5 = w[:, 4].astype(np.int64), w[:, 5].astype(np.int64)
+    b = (w5 >> 24) + ((w4 & 0xFFFFFF) << 8)
+    clk_total = int(t_ref[1]) + (b - b_ref) * SWARM_BFRAME_CLK
+    sec = int(t_ref[0]) + clk_total // SWARM_CLK_PER_SEC
+    clk = clk_total % SWARM_CLK_PER_SEC
+    b53 = (sec << 29) | clk
+    cid, fid = w5 & 0xFF, (w5 >> 16) & 0x7
+    w[:, 4] = (b53 >> 21).astype(np.uint32)
+    w[:, 5] = (cid | (fid << 8) | ((b53 & 0x1FFFFF) << 11)).astype(np.uint32)
+    rec = np.zeros((n // 8, 1056 + 7 * 1064), dtype=np.uint8)
+    k = np.arange(n // 8, dtype=np.int64)
+    first = w[0::8].copy()
+    first[:, 1] = (first[:, 1] & np.uint32(0xFF000000)) | (k & 0xFFFFFF).astype(np.uint32)
+    first[:, 2] = (first[:, 2] & np.uint32(0xFF000000)) | np.uint32((1056 + 7 * 1064) // 8)
+    first[:, 6] = (k & 0xFFFFFFFF).astype(np.uint32)
+    first[:, 7] = 0
+    first[:, 0] &= np.uint32(0x7FFFFFFF)
+    rec[:, :1056] = first.view(np.uint8)
+    for i in range(1, 8):
+        s = 1056 + (i - 1) * 1064 + 8
+        rec[:, s:s + 1056] = w[i::8].view(np.uint8)
+    return rec
+
+
+def make_stream_packets(seed, n_groups, frames=39, drop_frac=0.001, b_first=1000):
+    """A received stream for the input thread: B-frames b_first ..
+    b_first + frames n_groups in order (the first one is before the start the
+    thread derives and is skipped by it), packets shuffled within each B-frame,
+    round(drop_frac n) seed-chosen packets dropped (never the stream's first,
+    the count a multiple of 8).  Returns (packets in arrival order, the per-pol
+    codes of all frames)."""
+    rng = rng_for(seed)
+    n_frames = 1 + frames * n_groups
+    n_snap = n_frames * N_SNAP_PER_FRAME
+    c0 = gaussian_codes(rng, n_snap)
+    c1 = gaussian_codes(rng, n_snap)
+    pk = packetize(c0, c1, b_first)
+    order = np.concatenate([f * 2048 + rng.permutation(2048) for f in range(n_frames)])
+    n_drop = int(round(drop_frac * pk.shape[0]))
+    n_drop -= n_drop % 8
+    keep = np.ones(pk.shape[0], dtype=bool)
+    if n_drop:
+        keep[1 + rng.choice(pk.shape[0] - 1, n_drop, replace=False)] = False
+    order = order[keep[order]]
+    return np.ascontiguousarray(pk[order]), (c0, c1)

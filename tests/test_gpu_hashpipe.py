@@ -81,6 +81,63 @@ def test_plugin_two_groups_match_cabi():
     st.close()
 
 
+@pytest.mark.parametrize("variant,trim_mhz", [("trim6", 150), ("trim8", 102)])
+def test_plugin_band_trimmed_build_matches_cabi(variant, trim_mhz):
+    """A plugin built for the reference thread's band-trimmed path
+    (-DSDBE_PLUGIN_TRIMMED=1 at SWARM_RATE_FRAC 6 or 8): groups of
+    DECIMATION_FACTOR B-frames, the trim read from the status key TRIMDC (the
+    reference's trim_from_dc, vdif_inout_gpu_thread.cu:770-784), and
+    N_32bit_words_per_chan = 2 * 16384 * 128 * E / 16 (:1012).  Output equals
+    the C-ABI's band-trimmed handle on the same groups, byte for byte."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from sdbe_amd import _lib as L
+    from sdbe_amd import databuf as D
+    from sdbe_amd.stage import SdbeStage
+
+    _, rate, _, frames, slots = D.VARIANTS[variant]
+    D_, E = L.TRIM_PRESETS[rate]
+    assert D_ == frames
+    _say("[plugin-%s] generating 2 groups of %d B-frames" % (variant, frames))
+    groups, _ = S.make_group_packets(seed=65 + rate, n_groups=2, drop_frac=0.001, frames=frames)
+    dpk = [SdbeStage.to_device_packets(pk) for pk, _ in groups]
+    tmpl = O.fill_header_template(groups[0][0][0, :8], (100, 143_000_000))
+    Bgc = D.group_completion_type(frames)
+    hp = D.load(variant)
+    hp.hpc_status_clear()
+    assert hp.hpc_status_puti4(b"TRIMDC", trim_mhz) == 0
+    assert hp.hpc_start(b"vdif_inout_gpu_thread") == 0
+    words = 2 * 16384 * 128 * E // 16
+    try:
+        for i, (pk, b0) in enumerate(groups):
+            bgc = Bgc.from_address(hp.hpc_in_block(i))
+            bgc.beng_group_vdif_packet_count = pk.shape[0]
+            bgc.bgv_buf_gpu = dpk[i].data_ptr()
+            bgc.gpu_id = 0
+            for f in range(frames):
+                bgc.bfc[f].b = b0 + f
+            C.memmove(C.addressof(bgc.vdif_header_template), tmpl.ctypes.data, 32)
+            assert hp.hpc_set_in_filled(i) == 0
+        got = []
+        for j in range(2):
+            _say("[plugin-%s] waiting for output slot %d" % (variant, j))
+            assert hp.hpc_wait_out_filled(j, 120000) == 0
+            qs = D.QuantizedStorage.from_address(hp.hpc_out_block(j))
+            assert qs.bit_depth == 2 and qs.N_32bit_words_per_chan == words
+            got.append(_d2h(qs.vdg_buf_gpu, 2 * slots * 8192).reshape(2, slots, 8192))
+    finally:
+        hp.hpc_stop()
+    st = SdbeStage(policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, rate_frac=rate, skip_mhz=trim_mhz)
+    for i, (pk, b0) in enumerate(groups):
+        want = st.process_group(dpk[i], pk.shape[0], b0).cpu().numpy()
+        n = 128 * E
+        assert want.shape == (2, n, 8192)
+        assert np.array_equal(got[i][:, :n], want), i
+        _say("[plugin-%s] group %d: %d frames per polarisation equal the C-ABI's" % (variant, i, n))
+    st.close()
+
+
 def test_host_to_host_chain_through_databuf_copies():
     """The input thread's side (host bookkeeping into pinned group buffers,
     HIP host-to-device copy), the plugin, and the output thread's side (HIP

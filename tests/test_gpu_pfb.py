@@ -23,7 +23,7 @@ import pytest
 
 from oracle import sdbe_oracle as O
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("error::numpy.exceptions.ComplexWarning")]
 
 LB = 32768
 
@@ -34,8 +34,14 @@ def _say(msg):
 
 
 def _snr(ref, got):
-    ref = np.asarray(ref, dtype=np.float64).ravel()
-    err = np.asarray(got, dtype=np.float64).ravel() - ref
+    """SNR of got against ref; complex arrays are compared as complex (both
+    the real and the imaginary parts enter), real ones as real."""
+    ref = np.asarray(ref)
+    got = np.asarray(got)
+    assert ref.shape == got.shape and np.iscomplexobj(ref) == np.iscomplexobj(got)
+    dt = np.complex128 if np.iscomplexobj(ref) else np.float64
+    ref = ref.astype(dt).ravel()
+    err = got.astype(dt).ravel() - ref
     return 10.0 * math.log10(np.sum(np.abs(ref) ** 2) / max(np.sum(np.abs(err) ** 2), 1e-300))
 
 

@@ -157,3 +157,35 @@ def test_ts_offset_counts_frames_and_caps():
         if k == 0:
             want = 0
         assert off == want, (then, now, k, sign)
+
+
+def test_reference_policy_relatches_while_sigma_is_zero():
+    """The REFERENCE threshold walk (vdif_inout_gpu_thread.cu:954-1005): the
+    latch is taken from polarisation 0 and recomputed while the latched
+    threshold is 0.  Block 0 has an empty pol 0 (sigma 0) and a noisy pol 1;
+    block 1 has noise in both; block 2 has noise of another scale.
+    Expected, from the rule alone:
+    * block 0: th = off = 0 for both pols, so pol 0 (all zero) is code 3 and
+      pol 1 is 3 where x >= 0, 0 where x < 0;
+    * blocks 1 and 2: numpy's mean and std(ddof=1) of block 1's pol-0 samples.
+    Latching from pol 1, never recomputing, or recomputing every block fails."""
+    rng = np.random.default_rng(5)
+    beng = np.zeros((2, 3 * O.DECIMATION, O.BENG_CHANNELS, 2), dtype=np.int8)
+    beng[1, :39] = rng.integers(-2, 2, size=(39, O.BENG_CHANNELS, 2))
+    beng[:, 39:78] = rng.integers(-2, 2, size=(2, 39, O.BENG_CHANNELS, 2))
+    beng[:, 78:] = 2 * rng.integers(-1, 1, size=(2, 39, O.BENG_CHANNELS, 2))
+    out = O.process_blocks(beng, [0, 1, 2], policy=O.THRESH_REFERENCE, latch=[None])
+    x01 = O.resample_block(O.block_spectra(beng[1], 0))
+    x10 = O.resample_block(O.block_spectra(beng[0], 1))
+    assert not np.any(out.samples[(0, 0)])
+    assert np.array_equal(out.codes[(0, 0)], np.full(x01.size, 3, np.uint8))
+    assert np.array_equal(out.codes[(0, 1)], np.where(x01 >= 0, 3, 0).astype(np.uint8))
+    th, off = float(np.std(x10, ddof=1)), float(np.mean(x10))
+    for blk in (1, 2):
+        for pol in (0, 1):
+            got_th, got_off = out.thresholds[(blk, pol)]
+            assert abs(got_th - th) <= 1e-12 * th and abs(got_off - off) <= 1e-12 * th
+    x21 = O.resample_block(O.block_spectra(beng[1], 2))
+    v = x21 - off
+    want = np.select([v >= th, v >= 0, v > -th], [3, 2, 1], 0).astype(np.uint8)
+    assert np.array_equal(out.codes[(2, 1)], want)

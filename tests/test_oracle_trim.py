@@ -83,5 +83,11 @@ def test_preset_geometry():
     assert O.trim_geometry(6, 150) == (39, 32, 9984, 8192, 600)
     assert O.trim_geometry(6, 102) == (39, 32, 9984, 8192, 408)
     assert O.trim_geometry(8, 150) == (13, 8, 3328, 2048, 150)
+    # 10/11: 4160 Msps, R = 16640 (250 kHz bins); 11/11: 4576 Msps, R = 36608 (125 kHz bins)
+    assert O.trim_geometry(10, 0) == (65, 64, 16640, 16384, 0)
+    assert O.trim_geometry(10, 32) == (65, 64, 16640, 16384, 128)
+    assert O.trim_geometry(11, 150) == (143, 128, 36608, 32768, 1200)
+    with pytest.raises(ValueError):
+        O.trim_geometry(10, 33)               # 132 + 8192 > 8320 bins
     with pytest.raises(ValueError):
         O.trim_geometry(6, 300)               # 1200 + 4096 > 4992 bins

@@ -311,6 +311,27 @@ int sdbe_profile_read(sdbe_handle_t *h, sdbe_kernel_time_t *out, int32_t max_rec
  * two events per launch of that kernel only.  SDBE_ERR_ARG for an unknown name. */
 int sdbe_profile_select(sdbe_handle_t *h, const char *kernel);
 
+/* ---- allocation-free variant of sdbe_process_spectra --------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The same work as sdbe_process_spectra on a handle created
+ * with *cfg (chunk_blockpols fixed at 2; max_*, use_graph, rate_frac other than
+ * SDBE_RATE_64_39 not accepted or ignored as stated) whose stream state is
+ * *state, without a handle: the pipeline's working memory is one fixed device
+ * array of at most SDBE_NOALLOC_SCRATCH_BYTES per GPU inside the library, held
+ * from the first call on that GPU until the process ends.  It runs on the
+ * current HIP device and on `stream`; the first call on a device also uploads
+ * the twiddle tables (synchronously).  No call allocates or frees memory.
+ * n_bp above SDBE_NOALLOC_MAX_BLOCKPOLS returns SDBE_ERR_CAPACITY.
+ * *state is advanced by 64 n_bp frames.  The REFERENCE latch starts from zero
+ * in every call (the call's first block-pol latches).  All calls on one GPU
+ * share its array: calls that overlap on one GPU (two host threads, or two
+ * streams without an ordering between them) race and give undefined output.  */
+#define SDBE_NOALLOC_MAX_BLOCKPOLS 64
+#define SDBE_NOALLOC_SCRATCH_BYTES 41943040   /* 40 MiB */
+int sdbe_process_spectra_noalloc(const sdbe_config_t *cfg, sdbe_stream_state_t *state,
+                                 const int8_t *d_beng, int32_t n_bp, void *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,8 @@ SDBE_ERR_NOMEM = -3
 SDBE_ERR_CAPACITY = -4
 SDBE_ERR_NODEV = -5
 SDBE_ERR_STATE = -6
+NOALLOC_MAX_BLOCKPOLS = 64       # SDBE_NOALLOC_MAX_BLOCKPOLS
+NOALLOC_SCRATCH_BYTES = 41943040  # SDBE_NOALLOC_SCRATCH_BYTES
 
 THRESH_REFERENCE = 0
 THRESH_PER_BLOCK = 1
@@ -158,6 +160,8 @@ PROTOTYPES = [
     ("sdbe_reset_latch", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("sdbe_set_output_alignment", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     ("sdbe_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("sdbe_process_spectra_noalloc", C.c_int,
+     [C.POINTER(Config), C.POINTER(StreamState), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     # include/sdbe_pfb.h
     ("sdbe_pfb_create", C.c_int, [C.c_int, C.c_int32, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
     ("sdbe_pfb_destroy", None, [C.c_void_p]),

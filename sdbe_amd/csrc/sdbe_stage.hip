@@ -30,6 +30,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1067,6 +1068,44 @@ __global__ void __launch_bounds__(256) k_lead_copy(const uint8_t *__restrict__ l
     if (i >= n_bytes) return;
     *reinterpret_cast<GAS uint32_t *>(gptr(base) + pol * pol_stride + hdr_bytes + i) =
         *reinterpret_cast<const GAS uint32_t *>(gptr(lead) + pol * OUT_FRAME_PAYLOAD + i);
+}
+
+// Descriptors of n_bp back-to-back block-pols (the table sdbe_process_spectra
+// builds on the host), written on the device: block-pol b reads its rows at
+// rows + b * row_stride bytes, writes its frames at out + b * out_stride, is
+// polarisation 0 of no flip, and its first frame lies fpb b frames after
+// (secs, df, psn), df wrapping at 125000 frames per second.
+struct SpecDescArgs {
+    const int8_t *rows;
+    uint8_t *out;
+    int64_t row_stride, out_stride;
+    uint64_t psn;
+    uint32_t secs, df, w1hi, w3, w4;
+    int32_t policy, fpb;
+    float fixed_th, fixed_off;
+};
+__global__ void __launch_bounds__(64) k_spectra_desc(SpecDescArgs a, int n_bp, BpDesc *__restrict__ bpd) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_bp) return;
+    const uint64_t n = (uint64_t)b * (uint64_t)a.fpb;
+    const uint64_t dft = (uint64_t)a.df + n;
+    BpDesc d;
+    d.rows = a.rows + (int64_t)b * a.row_stride;
+    d.out = a.out + (int64_t)b * a.out_stride;
+    d.psn = a.psn + n;
+    d.secs = a.secs + (uint32_t)(dft / 125000u);
+    d.df = (uint32_t)(dft % 125000u);
+    d.w1hi = a.w1hi;
+    d.w3 = a.w3;
+    d.w4 = a.w4;
+    d.flip = 0;
+    d.pol = 0;
+    d.policy = (int16_t)a.policy;
+    d.slot = 0;
+    d.fixed_th = a.fixed_th;
+    d.fixed_off = a.fixed_off;
+    d.blk = 0;
+    bpd[b] = d;
 }
 
 #include "sdbe_trim.hpp"
@@ -2160,3 +2199,144 @@ int sdbe_profile_read(sdbe_handle_t *h, sdbe_kernel_time_t *out, int32_t max_rec
 
 }  // extern "C"
 
+// ============================================ allocation-free spectra entry
+// sdbe_process_spectra_noalloc keeps the pipeline's working memory in one
+// file-scope device array, `scratch`, instead of a handle's allocations: the
+// chunk buffers of NA_CHUNK block-pols, the twiddle and slot tables, the
+// counters, the latch slots and the descriptors of up to
+// SDBE_NOALLOC_MAX_BLOCKPOLS block-pols.  Every GPU the library runs on holds
+// its own copy of the array until the process ends.  In host code `scratch`
+// names a host-side stand-in, never the device memory: its device address is
+// taken once per device with hipGetSymbolAddress (na_setup) and only that
+// address is handed to kernels and copies.
+namespace {
+constexpr int NA_CHUNK = 2;
+constexpr int NA_MAX_DEV = 64;
+constexpr size_t na_al(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NA_V = 0;
+constexpr size_t NA_ZF = NA_V + na_al((size_t)NA_CHUNK * NBLK * NCH * sizeof(float2));
+constexpr size_t NA_BC = NA_ZF + na_al((size_t)NA_CHUNK * MP * sizeof(float2));
+constexpr size_t NA_STAGE = NA_BC + na_al((size_t)NA_CHUNK * KHI * BC_ROW * sizeof(float2));
+constexpr size_t NA_PART = NA_STAGE + na_al((size_t)NA_CHUNK * KHI * NCH);
+constexpr size_t NA_THR = NA_PART + na_al((size_t)NA_CHUNK * C1_TILES * 2 * sizeof(double));
+constexpr size_t NA_MS = NA_THR + na_al((size_t)NA_CHUNK * sizeof(float2));
+constexpr size_t NA_SLOT = NA_MS + na_al((size_t)NA_CHUNK * sizeof(double2));
+constexpr size_t NA_TAB = NA_SLOT + na_al((size_t)C2_SLOT_TAB * sizeof(int));
+constexpr size_t NA_CNT = NA_TAB + na_al(kTabEntries * sizeof(float2));
+constexpr size_t NA_LATCH = NA_CNT + na_al((size_t)CNT_N * sizeof(int));
+constexpr size_t NA_BPD = NA_LATCH + na_al((size_t)2 * SDBE_LATCH_SLOTS * sizeof(float));
+constexpr size_t NA_BYTES = NA_BPD + na_al((size_t)SDBE_NOALLOC_MAX_BLOCKPOLS * sizeof(BpDesc));
+static_assert(NA_BYTES <= SDBE_NOALLOC_SCRATCH_BYTES, "no-alloc scratch exceeds its stated size");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+struct NaDevice {
+    int ready;
+    int n_cu;
+    unsigned char *ptr;   // device address of `scratch` on this device
+};
+NaDevice g_na[NA_MAX_DEV];
+std::mutex g_na_mu;
+// host images of the tables, filled once (static storage: no allocation)
+float2 g_na_tab_host[kTabEntries];
+int g_na_slot_host[C2_SLOT_TAB];
+int g_na_seen_host[NCH];
+
+// Once per device, with that device current: the address of `scratch` and its
+// tables.  Returns the device's record or sets *rc.
+const NaDevice *na_setup(int dev, int *rc) {
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    NaDevice &d = g_na[dev];
+    if (d.ready) return &d;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        *rc = SDBE_ERR_HIP;
+        return nullptr;
+    }
+    unsigned char *base = (unsigned char *)p;
+    tables_fill_host(g_na_tab_host);
+    if (!slot_table_fill_host(g_na_slot_host, g_na_seen_host)) {
+        *rc = SDBE_ERR_STATE;
+        return nullptr;
+    }
+    if (hipMemcpy(base + NA_TAB, g_na_tab_host, kTabEntries * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(base + NA_SLOT, g_na_slot_host, C2_SLOT_TAB * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        *rc = SDBE_ERR_HIP;
+        return nullptr;
+    }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+    d.n_cu = ncu;
+    d.ptr = base;
+    d.ready = 1;
+    return &d;
+}
+}  // namespace
+
+extern "C" int sdbe_process_spectra_noalloc(const sdbe_config_t *cfg, sdbe_stream_state_t *state,
+                                            const int8_t *d_beng, int32_t n_bp, void *d_out, void *stream) {
+    if (!cfg || !state || !d_beng || !d_out || n_bp <= 0) return SDBE_ERR_ARG;
+    if (n_bp > SDBE_NOALLOC_MAX_BLOCKPOLS) return SDBE_ERR_CAPACITY;
+    if (cfg->thresh_policy < SDBE_THRESH_REFERENCE || cfg->thresh_policy > SDBE_THRESH_FIXED) return SDBE_ERR_ARG;
+    if (cfg->out_mode != SDBE_OUT_PAYLOAD && cfg->out_mode != SDBE_OUT_FRAMES) return SDBE_ERR_ARG;
+    if (cfg->c1_tiles_per_wg < 0 || cfg->c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
+    if (cfg->fft_precision < SDBE_FFT_FP32 || cfg->fft_precision > SDBE_FFT_FP16) return SDBE_ERR_ARG;
+    if (cfg->rate_frac != SDBE_RATE_64_39) return SDBE_ERR_ARG;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEV) return SDBE_ERR_NODEV;
+    int rc = SDBE_OK;
+    const NaDevice *nd = na_setup(dev, &rc);
+    if (!nd) return rc;
+    unsigned char *base = nd->ptr;
+    // a view of the pipeline's buffers inside `scratch` (a stack object: its
+    // profiling vectors stay empty and it allocates nothing)
+    sdbe_handle h;
+    h.device = dev;
+    h.cfg = *cfg;
+    h.chunk = NA_CHUNK;
+    h.c1_tpw = cfg->c1_tiles_per_wg ? cfg->c1_tiles_per_wg : C1_TILES_PER_WG;
+    h.n_cu = nd->n_cu;
+    h.tb = tables_bind((const float2 *)(base + NA_TAB));
+    h.V = (float2 *)(base + NA_V);
+    h.Zf = (float2 *)(base + NA_ZF);
+    h.Bc = (float2 *)(base + NA_BC);
+    h.stage = base + NA_STAGE;
+    h.part = (double *)(base + NA_PART);
+    h.thr = (float2 *)(base + NA_THR);
+    h.ms = (double2 *)(base + NA_MS);
+    h.slot_col = (int *)(base + NA_SLOT);
+    h.cnt = (int *)(base + NA_CNT);
+    h.latch = (float *)(base + NA_LATCH);
+    BpDesc *bpd = (BpDesc *)(base + NA_BPD);
+    hipStream_t s = (hipStream_t)stream;
+    const OutDesc od = out_desc(&h);
+    // the array keeps the previous call's contents: the counters and the
+    // REFERENCE latch (slot 0) start at zero in every call
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h.cnt, CNT_N, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int *)h.latch, 2, 0);
+    SpecDescArgs a;
+    a.rows = d_beng;
+    a.out = (uint8_t *)d_out;
+    a.row_stride = (int64_t)NBLK * NCH * 2;
+    a.out_stride = (int64_t)OUT_FRAMES_BLOCK * od.frame_bytes;
+    a.psn = state->psn;
+    a.secs = state->secs;
+    a.df = state->df;
+    a.w1hi = (state->ref_epoch & 0x3Fu) << 24;
+    a.w3 = (uint32_t)cfg->station_id | (1u << 26);
+    a.w4 = ((uint32_t)(cfg->bdc_sideband & 1u) << 1) | ((uint32_t)(cfg->rx_sideband & 1u) << 2) | (2u << 24);
+    a.policy = cfg->thresh_policy;
+    a.fpb = OUT_FRAMES_BLOCK;
+    a.fixed_th = cfg->fixed_threshold;
+    a.fixed_off = cfg->fixed_offset;
+    hipLaunchKernelGGL(k_spectra_desc, dim3((n_bp + 63) / 64), dim3(64), 0, s, a, n_bp, bpd);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return SDBE_ERR_HIP;
+    rc = run_resampler(&h, bpd, n_bp, od, false, nullptr, s);
+    if (rc) return rc;
+    advance_state(*state, (int64_t)n_bp * OUT_FRAMES_BLOCK);
+    return SDBE_OK;
+}

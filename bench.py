@@ -69,17 +69,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(groups, batch=32):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(groups, batch=32, single_core_bp=8):
     """The oracle's CPU chain (numpy/scipy FP64) on the whole benchmark
     workload: unpack of both groups (numpy, one thread), then 39 x C2R 32768,
     R2C 1,277,952, zero-pad C2R 2,097,152 (scipy.fft, parallel over a batch of
-    block-pols), sigma, quantise, pack, headers for all 512 block-pols."""
+    block-pols), sigma, quantise, pack, headers for all 512 block-pols.  Then
+    the same chain on one core (workers=1) over the first `single_core_bp`
+    block-pols, with the unpack charged at its measured time per block-pol."""
     import numpy as np
     import scipy
     from oracle import sdbe_oracle as O
-    workers = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    workers = max(1, min(avail, 16))       # the host share of one GPU
     t_unpack = t_chain = 0.0
     n_bp = 0
+    sample_X = None
     for pk, b0 in groups:
         t0 = time.perf_counter()
         un = O.vdif_to_beng(pk, b0)
@@ -87,6 +105,8 @@ def cpu_baseline(groups, batch=32):
         for blk0 in range(0, 128, batch // 2):
             blocks = range(blk0, blk0 + batch // 2)
             X = np.stack([O.block_spectra(un.beng[pol], blk) for blk in blocks for pol in (0, 1)])
+            if sample_X is None:
+                sample_X = X[:single_core_bp].copy()
             t0 = time.perf_counter()
             x = O.cpu_resample_batch(X, workers)
             for i in range(x.shape[0]):
@@ -97,16 +117,66 @@ def cpu_baseline(groups, batch=32):
             n_bp += x.shape[0]
         del un
     t = t_unpack + t_chain
+    # one core
+    t0 = time.perf_counter()
+    x = O.cpu_resample_batch(sample_X, 1)
+    for i in range(x.shape[0]):
+        mean, sig = O.block_stats(x[i])
+        O.pack_codes(O.quantize2bit(x[i], sig, mean))
+    O.stamp_headers(single_core_bp * 32, O.StreamState())
+    t1 = time.perf_counter() - t0 + t_unpack * single_core_bp / n_bp
     return {
         "value": n_bp * SAMPLES_PER_BP / t / 1e9,
         "unit": "Gsample/s",
         "cores": workers,
         "kind": "port",
-        "sample": ("the whole workload (%d block-pols): numpy unpack of 2 x 79,872-packet groups on "
-                   "1 thread %.1f s; scipy.fft FP64 C2R/R2C/C2R (workers=%d, batches of %d block-pols) + "
-                   "sigma + quantise + pack + headers %.1f s; numpy %s, scipy %s, FFTW absent"
-                   % (n_bp, t_unpack, workers, batch, t_chain, np.__version__, scipy.__version__)),
+        "single_core_value": single_core_bp * SAMPLES_PER_BP / t1 / 1e9,
+        "cpu_model": _cpu_model(),
+        "sample": ("the whole workload (%d block-pols) on %d of the host's %d cores: numpy unpack of "
+                   "2 x 79,872-packet groups on 1 thread %.1f s; scipy.fft FP64 C2R/R2C/C2R (workers=%d, "
+                   "batches of %d block-pols) + sigma + quantise + pack + headers %.1f s; one core: "
+                   "%d block-pols in %.1f s (workers=1, unpack at its measured share); numpy %s, scipy %s, "
+                   "FFTW absent"
+                   % (n_bp, workers, avail, t_unpack, workers, batch, t_chain, single_core_bp, t1,
+                      np.__version__, scipy.__version__)),
     }
+
+
+def per_block_pipeline(groups, dpk, precision=0, reps=2):
+    """The plain FP32 GPU pipeline launched per block, as the reference's run
+    loop does (software/aphids/vdif_inout_gpu_thread.cu:894-1008): per group
+    one unpack into the int8 rows, then 128 iterations of one block x 2
+    polarisations each (here one sdbe_process_blocks call of a one-block
+    stream on a chunk-2 handle: stage A, B, C1, statistics, threshold, C2,
+    pack, headers), the device synchronised after every iteration.  Returns
+    (input Gsample/s, ms per step of both groups)."""
+    import torch
+
+    from sdbe_amd import _lib as L
+    from sdbe_amd.stage import BatchStream, SdbeStage, process_blocks
+    st = SdbeStage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_FRAMES, chunk=2, station="Sm",
+                   fft_precision=precision)
+    out = torch.empty((2, 64, st.frame_bytes), dtype=torch.uint8, device="cuda")
+
+    def step():
+        for g, (pk, b0) in enumerate(groups):
+            b_0, b_1, _ = st.unpack_int8(dpk[g], pk.shape[0], b0)
+            torch.cuda.synchronize()
+            r0 = b_0.view(-1, 16384, 2)
+            r1 = b_1.view(-1, 16384, 2)
+            for blk in range(128):
+                s = BatchStream(r0[39 * blk:39 * (blk + 1)], r1[39 * blk:39 * (blk + 1)], out,
+                                policy=L.THRESH_PER_BLOCK, station="Sm", secs=1000, df=0, ref_epoch=30, psn=1)
+                process_blocks(st, [s])
+                torch.cuda.synchronize()
+
+    step()                                   # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    st.close()
+    return 2 * 2 * 4992 * 32768 / (ms * 1e-3) / 1e9, ms
 
 
 def main():
@@ -129,6 +199,8 @@ def main():
                          "2 overlaps the groups (about 3%% more throughput) but concurrent kernels share "
                          "the GPU, so per-kernel times and the roofline figure lose their meaning")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-block", action="store_true",
+                    help="skip the per-block (reference launch pattern) FP32 pipeline timed after the main run")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device copies of 256 MiB (the calibration workload of the "
@@ -299,6 +371,17 @@ def main():
         "kernels": kernels,
         "stage_stats": {k: stats[k] for k in ("n_unique", "n_missing", "n_duplicate", "n_invalid")},
     }
+    if not args.rate and not args.no_per_block:
+        log("[bench] per-block FP32 pipeline (one block x 2 pols per launch sequence, synchronised)")
+        pb_value, pb_ms = per_block_pipeline(groups, dpk)
+        rec["per_block_fp32"] = {
+            "value": round(pb_value, 3),
+            "ms_per_step": round(pb_ms, 3),
+            "what": ("the plain FP32 pipeline launched per block as the reference's run loop does "
+                     "(unpack per group, then 128 x one block of 2 pols, device synchronised after "
+                     "each), same workload"),
+            "ratio": round(value / pb_value, 2),
+        }
     if args.rate:
         rec["cpu_baseline"] = None       # the CPU leg times the 64/39 chain only
     elif not args.no_cpu:

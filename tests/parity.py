@@ -11,15 +11,15 @@ after the offset), in units of the block-pol's sigma:
 
 The FP32 pipeline's samples differ from FP64 by 3.7e-7 sigma RMS and at most
 about 2.5e-5 sigma (at the large samples the (code & 3) - 2 decode bias puts
-at the start of every snapshot).  About 0.9 of the samples per sigma lie near
-a boundary (pdf 0.40 at 0 and 0.24 at each of +-1 sigma), so an error e flips
-a code with probability about 0.9 e per sample: about one flip per block-pol
-in the inner shell and, from the error tail above 1e-6 sigma, an expected
-0.02 per block-pol in the middle shell.  The tests assert no outer-shell flip
-and at most MIDDLE_MAX_PER_BP middle-shell flips per block-pol (a rate 50x the
-expected one, so that a systematic error of 1e-6 sigma, which would flip
-about 2 codes per block-pol in that shell, is still caught across a group),
-and print all three counts.
+at the start of every snapshot).  Samples lie near a boundary with a measured
+probability density of about 1.85 per sigma, and an error e flips a code with
+probability about 1.85 e: from the measured errors that is about 1.2 flips per
+block-pol within 1e-6 sigma, 0.008 between 1e-6 and 1e-5 sigma and 3e-4 beyond
+(test_gpu_stage.py::test_resample_f32_vs_fp64 recomputes these).  The tests
+assert no outer-shell flip and at most MIDDLE_MAX_PER_BP middle-shell flips per
+block-pol (over 100 times the expected rate, yet an error of 2e-6 sigma on
+every sample, which would put about 4 flips per block-pol in that shell,
+fails), and print all three counts.
 """
 import numpy as np
 

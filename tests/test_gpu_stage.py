@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import sdbe_oracle as O
-from parity import ShellTally, boundary_distance
+from parity import MIDDLE_MAX_PER_BP, ShellTally, boundary_distance
 from sdbe_amd import synth as S
 
 pytestmark = pytest.mark.gpu
@@ -123,18 +123,19 @@ def test_resample_f32_vs_fp64(torch_cuda):
         mx = np.abs(err).max() / sig
         # expected code flips per shell from this error distribution: a sample
         # with error e flips its code when its distance d to a boundary is below
-        # |e|; distances near a boundary are uniform at the density rho per sigma
-        # measured here, so the expected flips with d in (a, b] are
-        # rho * sum(clip(|e|, a, b) - a)
+        # |e|; distances near a boundary are uniform at the probability density
+        # rho per sigma measured here, so the expected flips with d in (a, b]
+        # are rho * sum(clip(|e|, a, b) - a)
         d = boundary_distance(x64, sig, mean, sig)
-        rho = np.count_nonzero(d <= 1e-2) / 1e-2
+        rho = np.count_nonzero(d <= 1e-2) / 1e-2 / x64.size
         ae = np.abs(err) / sig
         exp_inner = rho * np.sum(np.clip(ae, 0.0, 1e-6))
         exp_mid = rho * np.sum(np.clip(ae, 1e-6, 1e-5) - 1e-6)
         exp_out = rho * np.sum(np.clip(ae, 1e-5, None) - 1e-5)
         _say("[resample] bp %d rms %.3g sigma, max %.3g sigma; boundary density %.3f / sigma; expected "
              "code flips per block-pol: %.3g within 1e-6 sigma, %.3g in 1e-6..1e-5, %.3g beyond"
-             % (b, rms, mx, rho / x64.size, exp_inner, exp_mid, exp_out))
+             % (b, rms, mx, rho, exp_inner, exp_mid, exp_out))
+        assert exp_mid <= 0.1 * MIDDLE_MAX_PER_BP and exp_out <= 0.01
         assert rms <= 2e-6
         assert mx <= 5e-5
         assert abs(stt[b, 0] - mean) <= 1e-6 * sig

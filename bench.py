@@ -80,13 +80,36 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(groups, batch=32, single_core_bp=8):
+def _code_shells(x, th, off, words_cpu, words_gpu):
+    """(within 1e-6 sigma, 1e-6..1e-5 sigma, beyond 1e-5 sigma) counts of the
+    2-bit codes that differ between two packed streams of the samples x, by
+    the distance of x from the nearest decision boundary (-th, 0, +th after the
+    offset) in units of th (the block-pol's sigma)."""
+    import numpy as np
+    from oracle import sdbe_oracle as O
+    bad_w = np.nonzero(words_cpu != words_gpu)[0]
+    if bad_w.size == 0:
+        return 0, 0, 0
+    cc = O.unpack_codes(words_cpu[bad_w]).reshape(-1, 16)
+    cg = O.unpack_codes(words_gpu[bad_w]).reshape(-1, 16)
+    wi, li = np.nonzero(cc != cg)
+    v = x[16 * bad_w[wi] + li] - off
+    d = np.min(np.abs(np.stack([v - th, v, v + th])), axis=0) / th
+    return int((d <= 1e-6).sum()), int(((d > 1e-6) & (d <= 1e-5)).sum()), int((d > 1e-5).sum())
+
+
+def cpu_baseline(groups, batch=32, single_core_bp=8, gpu_frames=None):
     """The oracle's CPU chain (numpy/scipy FP64) on the whole benchmark
     workload: unpack of both groups (numpy, one thread), then 39 x C2R 32768,
     R2C 1,277,952, zero-pad C2R 2,097,152 (scipy.fft, parallel over a batch of
     block-pols), sigma, quantise, pack, headers for all 512 block-pols.  Then
     the same chain on one core (workers=1) over the first `single_core_bp`
-    block-pols, with the unpack charged at its measured time per block-pol."""
+    block-pols, with the unpack charged at its measured time per block-pol.
+
+    With `gpu_frames` (per group, the stage's FRAMES output on the host,
+    [2 pols][8192 frames][8224 B], PER_BLOCK thresholds), the chain's codes
+    are also compared with the GPU's, outside the timed parts: differing codes
+    are counted by distance from a threshold."""
     import numpy as np
     import scipy
     from oracle import sdbe_oracle as O
@@ -98,7 +121,9 @@ def cpu_baseline(groups, batch=32, single_core_bp=8):
     t_unpack = t_chain = 0.0
     n_bp = 0
     sample_X = None
-    for pk, b0 in groups:
+    shells = [0, 0, 0]
+    n_cmp = 0
+    for g, (pk, b0) in enumerate(groups):
         t0 = time.perf_counter()
         un = O.vdif_to_beng(pk, b0)
         t_unpack += time.perf_counter() - t0
@@ -109,12 +134,23 @@ def cpu_baseline(groups, batch=32, single_core_bp=8):
                 sample_X = X[:single_core_bp].copy()
             t0 = time.perf_counter()
             x = O.cpu_resample_batch(X, workers)
+            packed = []
             for i in range(x.shape[0]):
                 mean, sig = O.block_stats(x[i])
-                O.pack_codes(O.quantize2bit(x[i], sig, mean))
+                packed.append((mean, sig, O.pack_codes(O.quantize2bit(x[i], sig, mean))))
             O.stamp_headers(len(blocks) * 64, O.StreamState())
             t_chain += time.perf_counter() - t0
             n_bp += x.shape[0]
+            if gpu_frames is not None:
+                for i, (mean, sig, words) in enumerate(packed):
+                    blk, pol = blocks[i // 2], i % 2
+                    fr = gpu_frames[g][pol, 64 * blk:64 * (blk + 1), 32:]
+                    wg = np.ascontiguousarray(fr).view("<u4").reshape(-1)
+                    a, b, c = _code_shells(x[i], sig, mean, words, wg)
+                    shells[0] += a
+                    shells[1] += b
+                    shells[2] += c
+                    n_cmp += 1
         del un
     t = t_unpack + t_chain
     # one core
@@ -125,7 +161,7 @@ def cpu_baseline(groups, batch=32, single_core_bp=8):
         O.pack_codes(O.quantize2bit(x[i], sig, mean))
     O.stamp_headers(single_core_bp * 32, O.StreamState())
     t1 = time.perf_counter() - t0 + t_unpack * single_core_bp / n_bp
-    return {
+    rec = {
         "value": n_bp * SAMPLES_PER_BP / t / 1e9,
         "unit": "Gsample/s",
         "cores": workers,
@@ -140,6 +176,15 @@ def cpu_baseline(groups, batch=32, single_core_bp=8):
                    % (n_bp, workers, avail, t_unpack, workers, batch, t_chain, single_core_bp, t1,
                       np.__version__, scipy.__version__)),
     }
+    if gpu_frames is not None:
+        rec["codes_vs_gpu"] = {
+            "block_pols": n_cmp,
+            "samples": n_cmp * 2097152,
+            "differ_within_1e-6_sigma": shells[0],
+            "differ_1e-6_to_1e-5_sigma": shells[1],
+            "differ_beyond_1e-5_sigma": shells[2],
+        }
+    return rec
 
 
 def per_block_pipeline(groups, dpk, precision=0, reps=2):
@@ -190,10 +235,13 @@ def main():
     ap.add_argument("--policy", default="per_block", choices=["per_block", "reference"])
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1, 2, 3],
                     help="FFT arithmetic: 0 FP32 (default), 1 BF16X3, 2 BF16, 3 FP16 matrix-core passes")
-    ap.add_argument("--rate", type=int, default=0, choices=[0, 6, 8],
-                    help="resampler: 0 the 64/39 path (default), 6 / 8 the aphids band-trimmed path "
-                         "of the 6/11 or 8/11 SWARM rate")
-    ap.add_argument("--skip-mhz", type=int, default=150, help="band-trimmed path: MHz skipped")
+    ap.add_argument("--rate", type=int, default=0, choices=[0, 6, 8, 10, 11],
+                    help="resampler: 0 the 64/39 path (default), 6 / 8 / 10 / 11 the aphids band-trimmed "
+                         "path of the 6/11, 8/11, 10/11 or 11/11 SWARM rate (groups of D = 39, 13, 65 "
+                         "or 143 B-frames)")
+    ap.add_argument("--skip-mhz", type=int, default=None,
+                    help="band-trimmed path: MHz skipped from DC (default 150; 16 at 10/11, whose "
+                         "band leaves room for at most 32)")
     ap.add_argument("--streams", type=int, default=1, choices=[1, 2],
                     help="HIP streams (one stage handle each); the 2 groups of a step alternate over them. "
                          "2 overlaps the groups (about 3%% more throughput) but concurrent kernels share "
@@ -213,11 +261,19 @@ def main():
         a.fill_(1)
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        for _ in range(3):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        for i in range(3):
             b.copy_(a)
+            ev[i + 1].record()
         torch.cuda.synchronize()
-        log("[bench] copied 3 x %d bytes" % n)
+        # the first copy warms up; the last two give the device copy rate
+        ms = ev[1].elapsed_time(ev[3]) / 2
+        log("[bench] copied 3 x %d bytes: %.1f GB/s (read + write) per copy after the first"
+            % (n, 2 * n / (ms * 1e-3) / 1e9))
         return
+    if args.skip_mhz is None:
+        args.skip_mhz = 16 if args.rate == 10 else 150
     if args.gpus != 1:
         print("bench.py measures one MI355X: every GPU box here has one GPU, and blocks are "
               "independent (N GPUs = N disjoint block ranges, no exchange); run with --gpus 1",
@@ -231,9 +287,6 @@ def main():
     from sdbe_amd import synth
     from sdbe_amd.stage import SdbeStage
 
-    log("[bench] generating 2 groups of packets (seed %d)" % args.seed)
-    groups, _ = synth.make_group_packets(seed=args.seed, n_groups=2, drop_frac=0.001)
-    n_pkts = sum(pk.shape[0] for pk, _ in groups)
     policy = L.THRESH_PER_BLOCK if args.policy == "per_block" else L.THRESH_REFERENCE
     # one handle (own buffers, own VDIF time state) per stream; group g of a
     # step runs on stream g mod streams
@@ -242,15 +295,23 @@ def main():
                      c1_tiles_per_wg=args.c1_tiles)
            for _ in range(args.streams)]
     st = sts[0]
+    # a group is 128 blocks of st.snaps snapshots: 39 B-frames at 64/39, D band-trimmed
+    frames = st.group_frames
+    log("[bench] generating 2 groups of %d B-frames (seed %d)" % (frames, args.seed))
+    groups, _ = synth.make_group_packets(seed=args.seed, n_groups=2, drop_frac=0.001, frames=frames)
+    n_pkts = sum(pk.shape[0] for pk, _ in groups)
     streams = [torch.cuda.Stream() for _ in range(args.streams)]
     alg_bytes = dict(ALG_BYTES)
     swarm_gsps = 2.496
     if args.rate:
         # band-trimmed kernels, per block of D snapshots: int8 rows -> series ->
         # 128 segments of P samples -> 2-bit payload
-        d, e = {6: (39, 32), 8: (13, 8)}[args.rate]
+        d, e = st.snaps, st.frames
         p = 256 * e
-        alg_bytes.update({"k_trim_c2r": d * 16384 * 2 + d * 32768 * 4,
+        npk = 2048 * d          # packets of a group of D B-frames
+        alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4,
+                          "k_reorder": npk * 1024 + 2 * 128 * d * 32768,
+                          "k_trim_c2r": d * 16384 * 2 + d * 32768 * 4,
                           "k_trim_seg": d * 32768 * 4 + 128 * p * 4,
                           "k_trim_pack": 128 * p * 4 + 128 * p // 4,
                           "k_headers": e * 32, "k_stats": 128 * 16})
@@ -302,9 +363,9 @@ def main():
     stats = st.stats()
     prof = wprof if wprof else tprof
 
-    n_bp = 2 * 2 * (4992 // st.snaps)
+    n_bp = 2 * 2 * (frames * 128 // st.snaps)
     ms_step = (t1 - t0) * 1e3 / args.steps
-    value = 2 * 2 * 4992 * 32768 / (ms_step * 1e-3) / 1e9     # input samples, any resampler
+    value = 2 * 2 * frames * 128 * 32768 / (ms_step * 1e-3) / 1e9     # input samples, any resampler
     bp_per_s = n_bp / (ms_step * 1e-3)
 
     kernels = {}
@@ -340,9 +401,9 @@ def main():
             "workload": ("plain FP32 pipeline at full rate: 256 blocks x 2 pols x 16384 ch "
                          "(2 groups of 39 B-frames), shuffled packets, 0.1% dropped, VDIF frames out"
                          if not args.rate else
-                         "band-trimmed %d/11 path (skip %d MHz): 2 groups of 39 B-frames x 2 pols x 16384 ch, "
-                         "shuffled packets, 0.1%% dropped, VDIF frames out" % (args.rate, args.skip_mhz)),
-            "rate": ["64/39", None, None, None, None, None, "6/11 trimmed", None, "8/11 trimmed"][args.rate],
+                         "band-trimmed %d/11 path (skip %d MHz): 2 groups of %d B-frames x 2 pols x 16384 ch, "
+                         "shuffled packets, 0.1%% dropped, VDIF frames out" % (args.rate, args.skip_mhz, frames)),
+            "rate": "64/39" if not args.rate else "%d/11 trimmed" % args.rate,
             "block_pols_per_step": n_bp,
             "packets_per_step": int(n_pkts),
             "threshold_policy": args.policy,
@@ -386,7 +447,8 @@ def main():
         rec["cpu_baseline"] = None       # the CPU leg times the 64/39 chain only
     elif not args.no_cpu:
         log("[bench] cpu baseline")
-        rec["cpu_baseline"] = cpu_baseline(groups)
+        frames = [o.cpu().numpy() for o in outs] if args.policy == "per_block" else None
+        rec["cpu_baseline"] = cpu_baseline(groups, gpu_frames=frames)
     print(json.dumps(rec))
     for h in sts:
         h.close()

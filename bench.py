@@ -232,6 +232,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--c1-tiles", type=int, default=0,
                     help="tuning: stage-C1 tiles per workgroup (0: the library default)")
+    ap.add_argument("--c1-ilv", type=int, default=0,
+                    help="tuning: stage-C1 workgroups that take adjacent tiles at each step (0: the "
+                         "library default)")
     ap.add_argument("--policy", default="per_block", choices=["per_block", "reference"])
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1, 2, 3],
                     help="FFT arithmetic: 0 FP32 (default), 1 BF16X3, 2 BF16, 3 FP16 matrix-core passes")
@@ -292,7 +295,7 @@ def main():
     # step runs on stream g mod streams
     sts = [SdbeStage(policy=policy, out_mode=L.OUT_FRAMES, chunk=args.chunk, station="Sm",
                      fft_precision=args.precision, rate_frac=args.rate, skip_mhz=args.skip_mhz,
-                     c1_tiles_per_wg=args.c1_tiles)
+                     c1_tiles_per_wg=args.c1_tiles, c1_interleave=args.c1_ilv)
            for _ in range(args.streams)]
     st = sts[0]
     # a group is 128 blocks of st.snaps snapshots: 39 B-frames at 64/39, D band-trimmed
@@ -409,6 +412,7 @@ def main():
             "threshold_policy": args.policy,
             "chunk_blockpols": args.chunk,
             "c1_tiles_per_wg": args.c1_tiles or 8,
+            "c1_interleave": args.c1_ilv or 16,
             "streams": args.streams,
             "kernel_table": "warmup steps, every launch timed; roofline kernel timed in the timed steps",
             "fft_precision": ["fp32", "bf16x3", "bf16", "fp16"][args.precision],

@@ -142,7 +142,11 @@ typedef struct sdbe_config {
                                    SDBE_RATE_6_11 / _8_11 / _10_11 / _11_11    */
     int32_t skip_mhz;           /* band-trimmed path: MHz skipped at the bottom
                                    of the band (aphids_resampler_init skip_mhz) */
-    int32_t reserved[2];
+    int32_t c1_interleave;      /* tuning: stage-C1 workgroups that take adjacent
+                                   tiles at each step (1..64, 0 = 16); 1 = each
+                                   workgroup runs c1_tiles_per_wg consecutive
+                                   tiles                                       */
+    int32_t reserved;
 } sdbe_config_t;
 
 /* VDIF output time state for the next frame to be produced; both

@@ -68,7 +68,8 @@ class Config(C.Structure):
         ("fft_precision", C.c_int32),
         ("rate_frac", C.c_int32),
         ("skip_mhz", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("c1_interleave", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

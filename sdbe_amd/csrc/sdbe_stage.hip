@@ -54,7 +54,8 @@ constexpr int MAP_NONE = 0x7FFFFFFF;
 // (c = a + 39 (16 u + j)), 14 tiles per class, 39 classes.
 constexpr int C1_CLASS_TILES = 14;
 constexpr int C1_TILES = NBLK * C1_CLASS_TILES;   // 546
-constexpr int C1_TILES_PER_WG = 8;                 // default consecutive tiles per stage C1 workgroup
+constexpr int C1_TILES_PER_WG = 8;                 // default tiles per stage C1 workgroup
+constexpr int C1_INTERLEAVE = 16;                  // default stage C1 workgroups per group of adjacent tiles
 constexpr int BC_ROW = C1_TILES * 32;            // 17472 slots per row of the first-pass output
 constexpr int OUT_FRAME_PAYLOAD = 8192;
 constexpr int OUT_FRAMES_BLOCK = 64;
@@ -511,11 +512,15 @@ __device__ __forceinline__ void c1_column(int tile, int col, int &c, bool &valid
     dup = (col >= 16) && (cl == 0 || cl == NCH / 2);
 }
 
-// Each workgroup runs `tpw` consecutive tiles in (block-pol, a, u) order, and
-// the workgroups are contiguous per XCD: tiles u and u + 1 of a class share the
-// 128-byte lines at the ends of their runs, which the second finds in L2.  The
-// next tile's packed-spectrum values and column twiddles are loaded while the
-// current tile is computed.
+// Tiles in (block-pol, a, u) order: tiles u and u + 1 of a class share the
+// 128-byte lines at the ends of their runs.  Workgroups are contiguous per XCD,
+// and `ilv` consecutive ones form a group that covers ilv * tpw consecutive
+// tiles: at step i, workgroup r of the group runs tile r + ilv i, so the group's
+// workgroups (dispatched together on one XCD) read neighbouring tiles at the
+// same time and fetch the shared lines once into L2.  With ilv = 1 a workgroup
+// runs tpw consecutive tiles, and a shared line must survive in L2 until its
+// next tile.  The next tile's packed-spectrum values and column twiddles are
+// loaded while the current tile is computed.
 struct C1Tile {
     int bpl, tile, c;
     bool valid, dup;
@@ -556,7 +561,7 @@ __device__ __forceinline__ void c1_fetch(const float2 *__restrict__ Zf, const Ta
 
 __global__ void __launch_bounds__(256)
 k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__restrict__ part, Tables tb, int nb,
-           int tpw) {
+           int tpw, int ilv) {
     // sz (packed-spectrum tile) is dead once Y is formed and sc (first-pass
     // tile) is written after that barrier, so they share storage: 28 KB of LDS
     // per workgroup, five workgroups per CU.
@@ -568,8 +573,10 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
     __shared__ double red[2][4];
     const int t = threadIdx.x, col = t & 31, rw = t >> 5, pcol = col ^ 16;
     const int n_tiles = C1_TILES * nb;
-    const int first = xcd_contig((int)blockIdx.x, (int)gridDim.x) * tpw;
-    const int last = min(first + tpw, n_tiles);
+    // the grid is a whole number of groups; tiles past n_tiles are not run
+    const int wg = xcd_contig((int)blockIdx.x, (int)gridDim.x), grp = wg / ilv;
+    const int first = grp * ilv * tpw + (wg - grp * ilv);
+    const int n_run = first < n_tiles ? min(tpw, (n_tiles - 1 - first) / ilv + 1) : 0;
     if (t < 40) {
         rowm[t] = tw<-1>(tb.nm, (uint32_t)(NCH * t));
     } else if (t < 104) {
@@ -581,12 +588,14 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
     C1Tile cur;
     float2 z[5], cm, cl;
     // Parseval sums of the tiles run so far for the current block-pol: reduced
-    // across the workgroup once per block-pol (a workgroup's tiles span at most
-    // two), into the slot of that block-pol's last tile here; its other slots
-    // get zeros, so the per-tile slots still sum to the block-pol's total
+    // across the workgroup once per block-pol it touches, into the slot of that
+    // block-pol's last tile here; its other slots get zeros, so the per-tile
+    // slots still sum to the block-pol's total
     double acc_run = 0.0, dc_run = 0.0;
-    if (first < last) c1_fetch(Zf, tb, first, col, rw, cur, z, cm, cl);
-    for (int lt = first; lt < last; ++lt) {
+    if (n_run > 0) c1_fetch(Zf, tb, first, col, rw, cur, z, cm, cl);
+    for (int i = 0; i < n_run; ++i) {
+        const int lt = first + ilv * i;
+        const bool more = i + 1 < n_run;
         const C1Tile x = cur;
         const int c = x.c;
         const float2 cmc = cm, clc = cl;
@@ -597,7 +606,7 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
             sz[col][r] = ok ? z[it] : make_float2(0.f, 0.f);
         }
         __syncthreads();
-        if (lt + 1 < last) c1_fetch(Zf, tb, lt + 1, col, rw, cur, z, cm, cl);
+        if (more) c1_fetch(Zf, tb, lt + ilv, col, rw, cur, z, cm, cl);
         // Y rows 0..39 (zero above M') + Parseval
         double acc = 0.0, dc = 0.0;
 #pragma unroll
@@ -681,7 +690,7 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
         // the next tile writes sz, which is sc's storage: all waves are done with sc
         __syncthreads();
         double *o = part + ((size_t)x.bpl * C1_TILES + x.tile) * 2;
-        if (lt + 1 == last || (lt + 1) / C1_TILES != x.bpl) {   // workgroup-uniform
+        if (!more || (lt + ilv) / C1_TILES != x.bpl) {   // workgroup-uniform
             // wave reduce then cross-wave
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
@@ -934,7 +943,10 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             }
         } else {
             const float2 to = thr[bpl];
-            uint8_t *dst = stage + (size_t)row * NCH + te;
+            // the row's 16384 bytes as a buffer: element te + 1024 q at offset te
+            // (VGPR) + 1024 q (scalar), with no 64-bit address arithmetic per store
+            const __amdgpu_buffer_rsrc_t rd =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(stage + (size_t)row * NCH), 0, NCH, BUF_RSRC_WORD3);
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const float a = v[q].x * scale - to.y;
@@ -943,7 +955,7 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
                 // 1 if -th < v < 0, 0 if v <= -th; b's code in bits 2-3 (disjoint: XOR = OR)
                 const uint32_t pair = ((fabsf(a) >= to.x) ? 3u : 2u) ^ (a < 0.f ? 3u : 0u) ^
                                       ((fabsf(b) >= to.x) ? 12u : 8u) ^ (b < 0.f ? 12u : 0u);
-                dst[1024 * q] = (uint8_t)pair;
+                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)pair, rd, te, 1024 * q, 0);
             }
         }
         if (next >= n_rows) break;
@@ -1167,6 +1179,7 @@ struct sdbe_handle {
     hipEvent_t g_done = nullptr;
     int g_used = 0;
     int c1_tpw = 8;
+    int c1_ilv = 16;
     int n_cu = 256;                  // compute units of the device
     int prof_kid = -1;               // profile only this kernel kind (-1: all)
     // resampler: 64/39 (trim.D == 0) or the band-trimmed path of a rate preset
@@ -1471,9 +1484,11 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
                        xm, rows);
             }
         }
-        const int c1_wgs = (C1_TILES * nb + h->c1_tpw - 1) / h->c1_tpw;
+        // whole groups of c1_ilv workgroups, c1_tpw tiles each
+        const int per_grp = h->c1_tpw * h->c1_ilv;
+        const int c1_wgs = (C1_TILES * nb + per_grp - 1) / per_grp * h->c1_ilv;
         LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(c1_wgs), dim3(256), (const float2 *)h->Zf, (float *)h->Bc, h->part, h->tb,
-               nb, h->c1_tpw);
+               nb, h->c1_tpw, h->c1_ilv);
         LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
         LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(256), (const double2 *)h->ms, nb, bpd, h->latch, h->thr);
         if (f32) {
@@ -1685,6 +1700,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     if (cfg.chunk_blockpols < 2 || cfg.chunk_blockpols > 256 || (cfg.chunk_blockpols & 1)) return SDBE_ERR_ARG;
     if (cfg.max_spectra_blockpols < 1 || cfg.max_batch_blockpols < 0) return SDBE_ERR_ARG;
     if (cfg.c1_tiles_per_wg < 0 || cfg.c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
+    if (cfg.c1_interleave < 0 || cfg.c1_interleave > 64) return SDBE_ERR_ARG;
     if (cfg.fft_precision < SDBE_FFT_FP32 || cfg.fft_precision > SDBE_FFT_FP16) return SDBE_ERR_ARG;
     trim::Geometry tg;
     if (cfg.rate_frac != SDBE_RATE_64_39) {
@@ -1700,6 +1716,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     h->cfg = cfg;
     h->chunk = cfg.chunk_blockpols;
     h->c1_tpw = cfg.c1_tiles_per_wg ? cfg.c1_tiles_per_wg : C1_TILES_PER_WG;
+    h->c1_ilv = cfg.c1_interleave ? cfg.c1_interleave : C1_INTERLEAVE;
     h->trim = tg;
     if (tg.D) {
         // a group is D B-frames of 128 snapshots: 128 blocks of D snapshots
@@ -2283,6 +2300,7 @@ extern "C" int sdbe_process_spectra_noalloc(const sdbe_config_t *cfg, sdbe_strea
     if (cfg->thresh_policy < SDBE_THRESH_REFERENCE || cfg->thresh_policy > SDBE_THRESH_FIXED) return SDBE_ERR_ARG;
     if (cfg->out_mode != SDBE_OUT_PAYLOAD && cfg->out_mode != SDBE_OUT_FRAMES) return SDBE_ERR_ARG;
     if (cfg->c1_tiles_per_wg < 0 || cfg->c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
+    if (cfg->c1_interleave < 0 || cfg->c1_interleave > 64) return SDBE_ERR_ARG;
     if (cfg->fft_precision < SDBE_FFT_FP32 || cfg->fft_precision > SDBE_FFT_FP16) return SDBE_ERR_ARG;
     if (cfg->rate_frac != SDBE_RATE_64_39) return SDBE_ERR_ARG;
     int dev = -1;
@@ -2298,6 +2316,7 @@ extern "C" int sdbe_process_spectra_noalloc(const sdbe_config_t *cfg, sdbe_strea
     h.cfg = *cfg;
     h.chunk = NA_CHUNK;
     h.c1_tpw = cfg->c1_tiles_per_wg ? cfg->c1_tiles_per_wg : C1_TILES_PER_WG;
+    h.c1_ilv = cfg->c1_interleave ? cfg->c1_interleave : C1_INTERLEAVE;
     h.n_cu = nd->n_cu;
     h.tb = tables_bind((const float2 *)(base + NA_TAB));
     h.V = (float2 *)(base + NA_V);

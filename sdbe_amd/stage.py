@@ -33,7 +33,7 @@ class SdbeStage:
     def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=256, device=0,
                  station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0), use_graph=False,
                  c1_tiles_per_wg=0, max_batch=0, fft_precision=L.FFT_FP32, rate_frac=L.RATE_64_39,
-                 skip_mhz=0):
+                 skip_mhz=0, c1_interleave=0):
         self.lib = L.load()
         cfg = L.Config()
         L.check(self.lib.sdbe_default_config(C.byref(cfg)), "sdbe_default_config")
@@ -52,6 +52,7 @@ class SdbeStage:
         cfg.fft_precision = fft_precision
         cfg.rate_frac = rate_frac
         cfg.skip_mhz = skip_mhz
+        cfg.c1_interleave = c1_interleave
         self.cfg = cfg
         # geometry of a block-pol: snapshots in, samples and frames out
         # (include/sdbe_stage.h, SDBE_RATE_*)

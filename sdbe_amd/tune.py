@@ -10,9 +10,10 @@ spectra repeat a seeded 8-block pattern (the pipeline's cost does not depend
 on the values).
 
 The tuner times one sdbe_process_blocks call over the whole batch for every
-(chunk, stage-C1 tiles per workgroup, HIP graph on/off) candidate -- the knobs
-that set how many block-pols share a launch and how many waves each CU holds
--- and reports the fastest.  Latency is reported two ways with the best
+(chunk, stage-C1 tiles per workgroup, stage-C1 interleave, HIP graph on/off)
+candidate -- the knobs that set how many block-pols share a launch, how much
+work each stage-C1 workgroup keeps in flight and which workgroups read
+neighbouring tiles together -- and reports the fastest.  Latency is reported two ways with the best
 configuration: per call for the whole batch, and per call when every stream
 hands in one block at a time (the latency-bound serving case).  Prints one
 JSON object per line.
@@ -94,6 +95,7 @@ def main():
     ap.add_argument("--seed", type=int, default=404)
     ap.add_argument("--chunks", default="32,64,128,256")
     ap.add_argument("--tiles", default="4,8,16")
+    ap.add_argument("--interleave", default="1,16")
     args = ap.parse_args()
     import torch
     from .stage import SdbeStage
@@ -103,14 +105,16 @@ def main():
     log("[tune] generating %d streams, %d block-pols" % (len(n_blocks), n_bp))
     data = make_batch(args.seed, n_blocks)
     results = []
-    for chunk, tiles, graph in itertools.product([int(x) for x in args.chunks.split(",")],
-                                                 [int(x) for x in args.tiles.split(",")], (False, True)):
-        st = SdbeStage(out_mode=L.OUT_FRAMES, chunk=chunk, c1_tiles_per_wg=tiles, use_graph=graph,
-                       max_batch=n_bp)
+    for chunk, tiles, ilv, graph in itertools.product([int(x) for x in args.chunks.split(",")],
+                                                      [int(x) for x in args.tiles.split(",")],
+                                                      [int(x) for x in args.interleave.split(",")], (False, True)):
+        st = SdbeStage(out_mode=L.OUT_FRAMES, chunk=chunk, c1_tiles_per_wg=tiles, c1_interleave=ilv,
+                       use_graph=graph, max_batch=n_bp)
         ms = time_calls(st, lambda: streams_for(st, data), args.reps)
         st.close()
         torch.cuda.empty_cache()
-        r = dict(chunk=chunk, c1_tiles_per_wg=tiles, graph=graph, batch_ms_p50=round(float(np.median(ms)), 3),
+        r = dict(chunk=chunk, c1_tiles_per_wg=tiles, c1_interleave=ilv, graph=graph,
+                 batch_ms_p50=round(float(np.median(ms)), 3),
                  gsample_s=round(n_bp * SAMPLES_PER_BP / (np.median(ms) * 1e-3) / 1e9, 2))
         log("[tune] %s" % r)
         results.append(r)
@@ -118,7 +122,7 @@ def main():
     print(json.dumps(dict(kind="autotune", streams=n_blocks, block_pols=n_bp, candidates=results, best=best)))
 
     st = SdbeStage(out_mode=L.OUT_FRAMES, chunk=best["chunk"], c1_tiles_per_wg=best["c1_tiles_per_wg"],
-                   use_graph=best["graph"], max_batch=n_bp)
+                   c1_interleave=best["c1_interleave"], use_graph=best["graph"], max_batch=n_bp)
     ms = time_calls(st, lambda: streams_for(st, data), max(args.reps, 20))
     print(json.dumps(dict(kind="latency_whole_batch", block_pols=n_bp, p50_ms=round(float(np.percentile(ms, 50)), 3),
                           p99_ms=round(float(np.percentile(ms, 99)), 3),

@@ -310,6 +310,24 @@ def test_process_group_order_invariant_and_deterministic(torch_cuda, one_group):
     st.close()
 
 
+@pytest.mark.parametrize("tiles,ilv,chunk", [(8, 16, 256), (3, 5, 6)])
+def test_process_group_c1_tile_order_does_not_change_output(torch_cuda, one_group, tiles, ilv, chunk):
+    """Stage C1's tile order (c1_tiles_per_wg x c1_interleave; the second case
+    leaves a partial last group in every chunk of 6 block-pols) changes only
+    which workgroup runs a tile and how the Parseval sums are grouped: the
+    group's output equals the one-workgroup-per-consecutive-run order."""
+    from sdbe_amd import _lib as L
+    pk, _, b0 = one_group
+    ref = _stage(policy=L.THRESH_PER_BLOCK, chunk=chunk, c1_tiles_per_wg=tiles, c1_interleave=1)
+    alt = _stage(policy=L.THRESH_PER_BLOCK, chunk=chunk, c1_tiles_per_wg=tiles, c1_interleave=ilv)
+    _say("[c1-order] tiles %d, interleave 1 vs %d, chunk %d" % (tiles, ilv, chunk))
+    oa = ref.process_group(ref.to_device_packets(pk), pk.shape[0], b0).cpu().numpy()
+    ob = alt.process_group(alt.to_device_packets(pk), pk.shape[0], b0).cpu().numpy()
+    assert np.array_equal(oa, ob)
+    ref.close()
+    alt.close()
+
+
 def test_process_group_packet_stage_a_matches_rows(torch_cuda, one_group):
     """sdbe_process_group's stage A decodes the packets in place (through the
     packet map, a missing packet as 0 + 0j); it must equal stage A over the

@@ -222,9 +222,11 @@ static void *run_method(hashpipe_thread_args_t *args) {
                 continue;
             }
             idx = 0;
+            cnt.all += n_pk;
             if (b_first == -1) {
                 while (idx < n_pk && (b_first = get_packet_b_count(&pk[idx].header)) < 0) ++idx;
-                if (b_first < 0) {   /* nothing valid yet: receive again */
+                if (b_first < 0) {   /* nothing valid yet (all invalid): receive again */
+                    cnt.invalid += n_pk;
                     free(pk);
                     pk = NULL;
                     n_pk = 0;
@@ -234,7 +236,6 @@ static void *run_method(hashpipe_thread_args_t *args) {
                     init_beng_group(&g_local.bgc[i], g_in.cpu[i], g_in.gpu[i],
                                     b_first + 1 + (int64_t)BENG_FRAMES_PER_GROUP * i);
             }
-            cnt.all += n_pk;
         }
         while (idx < n_pk) {
             const int off = get_beng_group_index_offset(&g_local, index_db_out, &pk[idx]);

@@ -35,9 +35,9 @@ ALG_BYTES = {
     "k_packet_map": 79872 * 32,                          # header sector per packet
     "k_reorder": 79872 * 1024 + 2 * 4992 * 32768,        # payloads in, int8 (im,re) rows out
     "k_stage_a": 39 * 16384 * 2 + 39 * 16384 * 8,        # int8 rows in, V out
-    "k_stage_a_pk": 39 * 16384 // 2 + 39 * 16384 * 8,    # the block-pol's 4 bits per channel in, V out
+    "k_stage_a_pk": 39 * 16384 // 2 + 39 * 16384 * 8,    # the block-pol's 4 bits per channel in, V (+ Zf row 0) out
     "k_group_bitmap": 79872 * 4 + 39 * 256,              # packet map in, fid bitmap out
-    "k_stage_b": 2 * 39 * 16384 * 8,                     # V in, packed R2C out
+    "k_stage_b": 2 * 38 * 16384 * 8,                     # V rows 1..38 in, packed R2C rows out
     "k_stage_c1": 39 * 16384 * 8 + (1 << 20) * 8,        # packed R2C in, first C2R pass out
     "k_stage_c2": (1 << 20) * 8 + (1 << 20),             # second pass in, 4-bit codes out
     "k_pack": (1 << 20) + 64 * 8192,                     # codes in, payloads out

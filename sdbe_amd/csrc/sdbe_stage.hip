@@ -14,8 +14,10 @@
 //   k_stage_a    decode + C2R pre-processing of each 32768-point snapshot
 //                transform + 39-point DFT across the block's snapshots
 //                (k_stage_a_pk: the same, decoding the group's packets in place)
-//   k_stage_b    per 39-DFT row: 16384-point inverse FFT, inter-pass twiddle,
-//                16384-point forward FFT  => packed 1,277,952-point R2C
+//   k_stage_b    per 39-DFT row k1 = 1..38: 16384-point inverse FFT, inter-pass
+//                twiddle, 16384-point forward FFT  => packed 1,277,952-point R2C
+//                (row 0, whose twiddle is 1, is 2^14 times stage A's row 0 and is
+//                written by stage A)
 //   k_stage_c1   R2C post-processing, zero padding, C2R pre-processing of the
 //                2,097,152-point output transform, first (64-point) pass of
 //                its packed 2^20 inverse FFT; FP64 Parseval partial sums
@@ -214,14 +216,20 @@ __device__ __forceinline__ float2 a_input(float Ra, float Ia, float Rb, float Ib
 
 typedef unsigned int u32v2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
 
-// V[bpl][k1][k] for k1 = 0..38: buffer stores, the row in the scalar offset,
+// V[bpl][k1][k] for k1 = 1..38: buffer stores, the row in the scalar offset,
 // non-temporal so that the streamed output does not evict the input lines
-// that the paired workgroup reads again.
-__device__ __forceinline__ void a_store(float2 *__restrict__ V, int bpl, int k, const float2 (&x)[NBLK]) {
+// that the paired workgroup reads again.  Row k1 = 0 goes straight to stage
+// B's output: its twiddle is 1, so stage B's inverse and forward 16384-point
+// transforms cancel (FFT(IFFT(x)) = 16384 x) and Zf's row 0 is 2^14 x (exact).
+__device__ __forceinline__ void a_store(float2 *__restrict__ V, float2 *__restrict__ Zf, int bpl, int k,
+                                        const float2 (&x)[NBLK]) {
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(V + (size_t)bpl * NBLK * NCH), 0, NBLK * NCH * (int)sizeof(float2), BUF_RSRC_WORD3);
+    const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(Zf + (size_t)bpl * NBLK * NCH), 0, NCH * (int)sizeof(float2), BUF_RSRC_WORD3);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32v2, cscale(x[0], 16384.0f)), rz, 8 * k, 0, CPOL_NT);
 #pragma unroll
-    for (int k1 = 0; k1 < NBLK; ++k1)
+    for (int k1 = 1; k1 < NBLK; ++k1)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32v2, x[k1]), rs, 8 * k,
                                               k1 * NCH * (int)sizeof(float2), CPOL_NT);
 }
@@ -236,7 +244,7 @@ __device__ __forceinline__ void a_store(float2 *__restrict__ V, int bpl, int k, 
 // channels N2 - k a workgroup reads are the ones its partner reads as k and
 // are served from that XCD's L2.
 __global__ void __launch_bounds__(256, 4)
-k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables tb) {
+k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb) {
     const int L = xcd_contig((int)blockIdx.x, (NCH / 256) * nb);
     const int bpl = L >> 6, idx = L & 63;
     const int w = (idx & 1) ? 63 - (idx >> 1) : (idx >> 1);
@@ -256,7 +264,7 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables
         x[s] = a_input(Ra, Ia, Rb, Ib, kz, make_float2(0.f, 0.f), make_float2(0.f, 0.f), T);
     }
     dft39<-1>(x);
-    a_store(V, bpl, k, x);
+    a_store(V, Zf, bpl, k, x);
 }
 
 // Stage A of a whole group, reading the packets in place of the int8 rows
@@ -276,7 +284,7 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables
 constexpr int APK_GA = 32, APK_GB = 33, APK_G = APK_GA + APK_GB;
 
 __global__ void __launch_bounds__(256, 2)
-k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tables tb,
+k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb,
              const uint32_t *__restrict__ pk, const int *__restrict__ map) {
     __shared__ uint2 seg[APK_G * NBLK];     // [group slot][snapshot] word pairs
     __shared__ int pidx[APK_G * 2];         // [group slot][frame f0, f1] packet index
@@ -331,7 +339,7 @@ k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, Tab
         x[s] = a_input(Ra, Ia, Rb, Ib, kz, cs, cd, T);
     }
     dft39<-1>(x);
-    a_store(V, bpl, k, x);
+    a_store(V, Zf, bpl, k, x);
 }
 
 // The fid bitmap byte of every (frame, cid) of the group and the count of
@@ -366,23 +374,25 @@ k_group_bitmap(const int *__restrict__ map, uint8_t *__restrict__ bitmap, int *_
 
 // Stage B: row k1 of block-pol bpl: w = IFFT16384(V[k1]); w[n] *= e^{-2 pi i n k1/638976};
 // Zf[k1 + 39 k2] = FFT16384(w)[k2]   (packed forward DFT of the 1,277,952 samples)
+// for k1 = 1..38 (stage A writes row 0, whose transforms cancel).
 // The packed spectrum is stored row-major: Zf[k1 + 39 k2] at k1 * 16384 + k2.
 // P = 0: FP32 VALU FFTs; P = FftMode: radix-16 passes on the matrix cores
 // (fft16k_mfma).
 // The workgroups are persistent (one per CU: the FFT's LDS fills a CU) and take
 // row slots lin = blockIdx.x, blockIdx.x + gridDim.x, ...  With xcd_map the
-// 39 rows of a block-pol all run on one XCD (workgroups are dealt round-robin
+// 38 rows of a block-pol all run on one XCD (workgroups are dealt round-robin
 // over the 8 XCDs; requires the chunk and the grid to be multiples of 8).  The
 // next row is loaded (buffer loads, no VALU address arithmetic) while the last
 // pass, the conjugation and the stores of the current row run.
+constexpr int B_ROWS = NBLK - 1;   // rows k1 = 1..38 of a block-pol
 __device__ __forceinline__ void b_row(int lin, int xcd_map, int &bpl, int &k1) {
     if (xcd_map) {
         const int slot = lin >> 3;
-        bpl = (lin & 7) + 8 * (slot / NBLK);
-        k1 = slot % NBLK;
+        bpl = (lin & 7) + 8 * (slot / B_ROWS);
+        k1 = 1 + slot % B_ROWS;
     } else {
-        bpl = lin / NBLK;
-        k1 = lin % NBLK;
+        bpl = lin / B_ROWS;
+        k1 = 1 + lin % B_ROWS;
     }
 }
 
@@ -1458,12 +1468,12 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
         if (pk)
-            LAUNCH(h, KID_A_PK, nb, s, k_stage_a_pk, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb, pk,
+            LAUNCH(h, KID_A_PK, nb, s, k_stage_a_pk, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->Zf, h->tb, pk,
                    (const int *)h->map);
         else
-            LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->tb);
+            LAUNCH(h, KID_A, nb, s, k_stage_a, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->Zf, h->tb);
         {
-            const int rows = NBLK * nb;
+            const int rows = B_ROWS * nb;
             const int grid = rows < h->n_cu ? rows : h->n_cu;
             const int xm = (nb % 8) == 0 && (grid % 8) == 0 ? 1 : 0;
             switch (h->cfg.fft_precision) {

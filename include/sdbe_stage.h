@@ -65,6 +65,8 @@ extern "C" {
 #define SDBE_OUT_PAYLOAD_BYTES   8192
 #define SDBE_OUT_HEADER_BYTES    32
 #define SDBE_OUT_FRAMES_PER_GROUP (SDBE_BLOCKS_PER_GROUP * SDBE_OUT_FRAMES_PER_BLOCK)
+#define SDBE_MAX_CHUNK           1024   /* block-pols per pipeline chunk, at most
+                                           (about 21 GB of working buffers at 64/39) */
 
 /* ---- threshold policy --------------------------------------------------- */
 /* REFERENCE: mean and ddof=1 sigma of the first block of polarisation 0 are
@@ -125,7 +127,9 @@ typedef struct sdbe_config {
     float   fixed_threshold;    /* used with SDBE_THRESH_FIXED                  */
     float   fixed_offset;
     int32_t out_mode;           /* SDBE_OUT_*                                   */
-    int32_t chunk_blockpols;    /* block-pols per pipeline chunk (even, 2..256) */
+    int32_t chunk_blockpols;    /* block-pols per pipeline chunk (even, 2..SDBE_MAX_CHUNK;
+                                   a batch of up to this many runs as one launch
+                                   per stage)                                  */
     int32_t max_spectra_blockpols; /* capacity of sdbe_process_spectra (>= 1)   */
     uint16_t station_id;        /* VDIF w3.stationID (2 ASCII chars, "XX")      */
     uint8_t  bdc_sideband;      /* VDIF w4.bdc_sideband                        */

@@ -770,7 +770,7 @@ k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, doub
 // (vdif_inout_gpu_thread.cu:954-1005).  One thread walks the chunk for it,
 // over the statistics, policy fields and latch slots staged in LDS, so that
 // no step of the walk waits on global memory; the slots go back at the end.
-constexpr int THRESH_MAX_CHUNK = 256;
+constexpr int THRESH_MAX_CHUNK = SDBE_MAX_CHUNK;
 __global__ void __launch_bounds__(256) k_thresh(const double2 *__restrict__ ms, int nbpl,
                                                 const BpDesc *__restrict__ bpd, float *__restrict__ latch,
                                                 float2 *__restrict__ thr) {
@@ -1707,7 +1707,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     else sdbe_default_config(&cfg);
     if (cfg.thresh_policy < SDBE_THRESH_REFERENCE || cfg.thresh_policy > SDBE_THRESH_FIXED) return SDBE_ERR_ARG;
     if (cfg.out_mode != SDBE_OUT_PAYLOAD && cfg.out_mode != SDBE_OUT_FRAMES) return SDBE_ERR_ARG;
-    if (cfg.chunk_blockpols < 2 || cfg.chunk_blockpols > 256 || (cfg.chunk_blockpols & 1)) return SDBE_ERR_ARG;
+    if (cfg.chunk_blockpols < 2 || cfg.chunk_blockpols > SDBE_MAX_CHUNK || (cfg.chunk_blockpols & 1)) return SDBE_ERR_ARG;
     if (cfg.max_spectra_blockpols < 1 || cfg.max_batch_blockpols < 0) return SDBE_ERR_ARG;
     if (cfg.c1_tiles_per_wg < 0 || cfg.c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
     if (cfg.c1_interleave < 0 || cfg.c1_interleave > 64) return SDBE_ERR_ARG;

@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--blocks", default="17,64,128,5,200,33")
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--seed", type=int, default=404)
-    ap.add_argument("--chunks", default="32,64,128,256")
+    ap.add_argument("--chunks", default="64,128,256,512,1024")
     ap.add_argument("--tiles", default="4,8,16")
     ap.add_argument("--interleave", default="1,16")
     args = ap.parse_args()

@@ -10,8 +10,9 @@ sdbe_process_blocks call captured as a HIP graph.
   byte-identical to the first.
 * The full shape {17, 64, 128, 5, 200, 33} (894 block-pols): each stream's
   spectra repeat a seeded 8-block pattern, so under PER_BLOCK thresholds block
-  j and block j + 8 of a stream carry byte-identical payloads; graph replays
-  (eager work in between) are byte-identical to the eager launch sequence;
+  j and block j + 8 of a stream carry byte-identical payloads; the whole batch
+  as one chunk (one launch per stage) captured as a graph, replayed with eager
+  work in between, is byte-identical to the eager run in chunks of 256;
   headers and state advance by 64 n_blocks frames bit-exact; block 0 of every
   stream matches the FP64 oracle.
 """
@@ -144,7 +145,8 @@ def test_config4_full_shape_properties(torch_cuda):
     torch.cuda.synchronize()
     eager.close()
     outs = [torch.empty_like(s.out) for s in ref]
-    g = SdbeStage(out_mode=L.OUT_FRAMES, use_graph=True)
+    # the whole batch as one chunk (one launch per stage), captured as a graph
+    g = SdbeStage(out_mode=L.OUT_FRAMES, use_graph=True, chunk=1024)
     for rep in range(3):
         for o in outs:
             o.fill_(0x5A)

@@ -236,6 +236,9 @@ def main():
                     help="tuning: stage-C1 workgroups that take adjacent tiles at each step (0: the "
                          "library default)")
     ap.add_argument("--policy", default="per_block", choices=["per_block", "reference"])
+    ap.add_argument("--geometry", default="reference", choices=["reference", "8x1024"],
+                    help="packet geometry: the reference's 128 snapshots x 8 channels (1056-byte packets) or "
+                         "8 snapshots x 1024 channels (8224-byte packets)")
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1, 2, 3],
                     help="FFT arithmetic: 0 FP32 (default), 1 BF16X3, 2 BF16, 3 FP16 matrix-core passes")
     ap.add_argument("--rate", type=int, default=0, choices=[0, 6, 8, 10, 11],
@@ -295,25 +298,33 @@ def main():
     # step runs on stream g mod streams
     sts = [SdbeStage(policy=policy, out_mode=L.OUT_FRAMES, chunk=args.chunk, station="Sm",
                      fft_precision=args.precision, rate_frac=args.rate, skip_mhz=args.skip_mhz,
-                     c1_tiles_per_wg=args.c1_tiles, c1_interleave=args.c1_ilv)
+                     c1_tiles_per_wg=args.c1_tiles, c1_interleave=args.c1_ilv,
+                     packet_geometry=L.GEOM_8X1024 if args.geometry == "8x1024" else L.GEOM_REFERENCE)
            for _ in range(args.streams)]
     st = sts[0]
     # a group is 128 blocks of st.snaps snapshots: 39 B-frames at 64/39, D band-trimmed
     frames = st.group_frames
     log("[bench] generating 2 groups of %d B-frames (seed %d)" % (frames, args.seed))
-    groups, _ = synth.make_group_packets(seed=args.seed, n_groups=2, drop_frac=0.001, frames=frames)
+    groups, _ = synth.make_group_packets(seed=args.seed, n_groups=2, drop_frac=0.001, frames=frames,
+                                         geometry=1 if args.geometry == "8x1024" else 0)
     n_pkts = sum(pk.shape[0] for pk, _ in groups)
     streams = [torch.cuda.Stream() for _ in range(args.streams)]
     alg_bytes = dict(ALG_BYTES)
     swarm_gsps = 2.496
+    # packets per B-frame and payload bytes per packet of the packet geometry
+    ppf, pay = (256, 8192) if args.geometry == "8x1024" else (2048, 1024)
+    if args.geometry == "8x1024":
+        npk = ppf * frames
+        alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4,
+                          "k_reorder": npk * pay + 2 * 128 * frames * 32768})
     if args.rate:
         # band-trimmed kernels, per block of D snapshots: int8 rows -> series ->
         # 128 segments of P samples -> 2-bit payload
         d, e = st.snaps, st.frames
         p = 256 * e
-        npk = 2048 * d          # packets of a group of D B-frames
+        npk = ppf * d           # packets of a group of D B-frames
         alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4,
-                          "k_reorder": npk * 1024 + 2 * 128 * d * 32768,
+                          "k_reorder": npk * pay + 2 * 128 * d * 32768,
                           "k_trim_c2r": d * 16384 * 2 + d * 32768 * 4,
                           "k_trim_seg": d * 32768 * 4 + 128 * p * 4,
                           "k_trim_pack": 128 * p * 4 + 128 * p // 4,
@@ -407,6 +418,9 @@ def main():
                          "band-trimmed %d/11 path (skip %d MHz): 2 groups of %d B-frames x 2 pols x 16384 ch, "
                          "shuffled packets, 0.1%% dropped, VDIF frames out" % (args.rate, args.skip_mhz, frames)),
             "rate": "64/39" if not args.rate else "%d/11 trimmed" % args.rate,
+            "packet_geometry": ("128 snapshots x 8 channels x 2 pols per 1056-byte packet (reference)"
+                                if args.geometry == "reference" else
+                                "8 snapshots x 1024 channels x 2 pols per 8224-byte packet"),
             "block_pols_per_step": n_bp,
             "packets_per_step": int(n_pkts),
             "threshold_policy": args.policy,

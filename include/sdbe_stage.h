@@ -117,6 +117,14 @@ extern "C" {
  * frames per polarisation); sdbe_process_blocks and sdbe_set_output_alignment
  * return SDBE_ERR_STATE.                                                    */
 #define SDBE_RATE_64_39 0
+/* packet geometry (sdbe_config_t.packet_geometry) */
+#define SDBE_GEOM_REFERENCE 0   /* 1056-byte packets: 128 snapshots x 8 channels x 2 pols,
+                                   2048 per B-frame (cid 0..255, fid 0..7)          */
+#define SDBE_GEOM_8X1024    1   /* 8224-byte packets: 8 snapshots x 1024 channels x 2 pols,
+                                   256 per B-frame (cid 0..15: channels 1024 cid ..,
+                                   fid 0..15: snapshots 8 fid ..), same header and
+                                   byte encoding; no counterpart in the reference      */
+
 #define SDBE_RATE_6_11  6
 #define SDBE_RATE_8_11  8
 #define SDBE_RATE_10_11 10
@@ -150,7 +158,8 @@ typedef struct sdbe_config {
                                    tiles at each step (1..64, 0 = 16); 1 = each
                                    workgroup runs c1_tiles_per_wg consecutive
                                    tiles                                       */
-    int32_t reserved;
+    int32_t packet_geometry;    /* SDBE_GEOM_*: the packets sdbe_process_group and
+                                   sdbe_unpack_group_int8 take                  */
 } sdbe_config_t;
 
 /* VDIF output time state for the next frame to be produced; both
@@ -203,12 +212,14 @@ int sdbe_stream_state_from_template(const void *template_header32,
                                     int32_t *skip_int32, int32_t *skip_frames);
 
 /* Process one group.  d_pkts: n_pkts packets of 1056 bytes (device), at
- * most the group's 39 * 2048 (band-trimmed: D * 2048).  b_zero: B-count of
+ * most the group's 39 * 2048 (band-trimmed: D * 2048); with SDBE_GEOM_8X1024
+ * packets of 8224 bytes, at most 39 * 256 (D * 256).  b_zero: B-count of
  * the group's first frame (bfc[0].b).  d_out: output, polarisation p at
  * d_out + p * out_pol_stride_bytes, 8192 frames (band-trimmed: 128 E) each of
  * 8192 (PAYLOAD) or 8224 (FRAMES) bytes.  d_bitmap (optional, device):
  * uint8 [frames][256], bit f of [frame][cid] set when packet (frame, cid, f)
- * arrived (the cc[] flags of vdif_in_databuf.h:112-130).                   */
+ * arrived (the cc[] flags of vdif_in_databuf.h:112-130); with
+ * SDBE_GEOM_8X1024 uint16 [frames][16], bit f of [frame][cid].            */
 int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
                        uint64_t b_zero, void *d_out, int64_t out_pol_stride_bytes,
                        uint8_t *d_bitmap, void *stream);

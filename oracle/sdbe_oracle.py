@@ -189,6 +189,55 @@ def vdif_to_beng(pkts, b_zero, n_frames=DECIMATION):
     )
 
 
+# The "8 snapshots x 1024 channels" packet geometry of the benchmark's
+# configuration text (no reference counterpart; the stage's SDBE_GEOM_8X1024).
+# Packet (frame, cid, fid), cid < 16, fid < 16: snapshots 8 fid .. 8 fid + 7 of
+# its B-frame, channels 1024 cid .. 1024 cid + 1023; payload row sn is 1024
+# bytes, byte q holding channel (q & ~3) + 3 - (q & 3) (the reference's order
+# within a 32-bit word), each byte decoded by decode_byte.  Header as the
+# reference's (cid, fid and the B-count in words 4-5).
+G2_CIDS, G2_FIDS, G2_SNAPS, G2_CHANS = 16, 16, 8, 1024
+G2_PKT_WORDS = HDR_WORDS + G2_SNAPS * G2_CHANS // 4
+G2_BYTE_TO_CHAN = (np.arange(G2_CHANS) & ~3) + 3 - (np.arange(G2_CHANS) & 3)
+
+
+def vdif_to_beng_8x1024(pkts, b_zero, n_frames=DECIMATION):
+    """Unpack one group of 8 x 1024 packets ((n, 2056) uint32) into the same
+    int8 buffers as ``vdif_to_beng``; the lowest-index packet of a slot wins,
+    missing slots stay 0 + 0j.  bitmap: uint16 [frame][cid], bit fid."""
+    pkts = np.ascontiguousarray(pkts, dtype=np.uint32)
+    hdr = in_header_fields(pkts[:, :HDR_WORDS])
+    cid, fid, rel = gpu_cid_fid_rel(pkts[:, :HDR_WORDS], b_zero)
+    invalid = hdr["invalid"] == 1
+    in_range = (rel >= 0) & (rel < n_frames) & (cid < G2_CIDS) & (fid < G2_FIDS)
+    ok = (~invalid) & in_range
+    idx_ok = np.nonzero(ok)[0]
+    key = (rel[idx_ok] * G2_CIDS + cid[idx_ok]) * G2_FIDS + fid[idx_ok]
+    _, first = np.unique(key, return_index=True)
+    sel = idx_ok[first]
+    beng = np.zeros((2, n_frames, BENG_SNAPSHOTS, BENG_CHANNELS, 2), dtype=np.int8)
+    bitmap = np.zeros((n_frames, G2_CIDS), dtype=np.uint16)
+    for p in sel:
+        payload = pkts[p, HDR_WORDS:].view(np.uint8).reshape(G2_SNAPS, G2_CHANS)
+        p0_re, p0_im, p1_re, p1_im = decode_byte(payload)
+        chans = G2_CHANS * cid[p] + G2_BYTE_TO_CHAN
+        rows = G2_SNAPS * fid[p] + np.arange(G2_SNAPS)
+        f = rel[p]
+        beng[0, f, rows[:, None], chans[None, :], 0] = p0_im
+        beng[0, f, rows[:, None], chans[None, :], 1] = p0_re
+        beng[1, f, rows[:, None], chans[None, :], 0] = p1_im
+        beng[1, f, rows[:, None], chans[None, :], 1] = p1_re
+        bitmap[f, cid[p]] |= np.uint16(1 << int(fid[p]))
+    return UnpackResult(
+        beng=beng, bitmap=bitmap,
+        n_invalid=int(invalid.sum()),
+        n_out_of_range=int(((~invalid) & (~in_range)).sum()),
+        n_duplicate=int(idx_ok.size - sel.size),
+        n_unique=int(sel.size),
+        n_missing=int(n_frames * G2_CIDS * G2_FIDS - sel.size),
+    )
+
+
 def inflate(beng_pol):
     """int8 (im, re) pairs -> complex spectra with the zeroed Nyquist channel
     16384 appended (vdif_inout_gpu_thread.cu:532-556 and the memset at :925-926)."""

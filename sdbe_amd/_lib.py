@@ -32,6 +32,9 @@ RATE_11_11 = 11
 # band-trimmed path (include/sdbe_stage.h); a group is D B-frames, 128 blocks
 # of D snapshots, E frames of 8192 payload bytes per block-pol
 TRIM_PRESETS = {RATE_6_11: (39, 32), RATE_8_11: (13, 8), RATE_10_11: (65, 64), RATE_11_11: (143, 128)}
+GEOM_REFERENCE = 0              # 1056-byte packets, 128 snapshots x 8 channels
+GEOM_8X1024 = 1                 # 8224-byte packets, 8 snapshots x 1024 channels
+PKT_WORDS_8X1024 = 2056
 FFT_FP32 = 0
 FFT_BF16X3 = 1
 FFT_BF16 = 2
@@ -69,7 +72,7 @@ class Config(C.Structure):
         ("rate_frac", C.c_int32),
         ("skip_mhz", C.c_int32),
         ("c1_interleave", C.c_int32),
-        ("reserved", C.c_int32),
+        ("packet_geometry", C.c_int32),
     ]
 
 

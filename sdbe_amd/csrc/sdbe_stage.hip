@@ -52,6 +52,13 @@ constexpr int MAP_N = NFR * 256 * 8;          // 79,872 packet slots per group
 // DECIMATION_FACTOR, sample_rates.h:67): at most 143 (11/11).
 constexpr int NFR_MAX = 143;
 constexpr int MAP_NONE = 0x7FFFFFFF;
+// Packet geometry SDBE_GEOM_8X1024 (the "8 snapshots x 1024 channels" packets
+// of the benchmark's configuration text; no reference counterpart): a packet
+// (frame, cid, fid) carries snapshots 8 fid .. 8 fid + 7 of its B-frame for
+// channels 1024 cid .. 1024 cid + 1023, one byte per channel and snapshot with
+// the reference's byte encoding; 256 packets per B-frame, 8224 bytes each.
+constexpr int G2_CIDS = 16, G2_FIDS = 16, G2_SNAPS = 8;
+constexpr int G2_PKT_WORDS = 8 + G2_SNAPS * NCH / G2_CIDS / 4;   // 2056
 // Stage C1 tiles: columns in residue classes mod 39, 16 per tile
 // (c = a + 39 (16 u + j)), 14 tiles per class, 39 classes.
 constexpr int C1_CLASS_TILES = 14;
@@ -64,6 +71,17 @@ constexpr int OUT_FRAMES_BLOCK = 64;
 
 enum { CNT_INVALID = 0, CNT_RANGE = 1, CNT_ACCEPTED = 2, CNT_UNIQUE = 3, CNT_N = 8 };
 
+// Pointers read from a descriptor are generic to the compiler, which would
+// then emit flat memory instructions (counted on both the vector-memory and
+// LDS counters, so that every load waits for the previous ones).  These casts
+// state that they address global memory.
+#define GAS __attribute__((address_space(1)))
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ GAS T *gptr(T *p) {
+    return (GAS T *)p;
+}
+
 // ---------------------------------------------------------------- unpack side
 __global__ void __launch_bounds__(256) k_fill_i32(int *p, int n, int val) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -72,26 +90,27 @@ __global__ void __launch_bounds__(256) k_fill_i32(int *p, int n, int val) {
 
 // One thread per packet: header words 0, 4, 5 give invalid, B-count, cid, fid
 // (vdif_inout_gpu_thread.cu:353-369,437 restated).  Valid packets inside the
-// group's window of nfr frames claim their map slot with atomicMin -> lowest
-// index wins.
+// group's window of nfr frames claim their map slot [frame][cid][fid] with
+// atomicMin -> lowest index wins.  Geometry: n_cid x n_fid packets per frame
+// of pkt_words words each (reference: 256 x 8 x 264; 8 x 1024: 16 x 16 x 2056).
 __global__ void __launch_bounds__(256)
-k_packet_map(const uint32_t *__restrict__ pk, int n, uint32_t bzero32, int nfr, int *__restrict__ map,
-             int *__restrict__ cnt) {
+k_packet_map(const uint32_t *__restrict__ pk, int n, uint32_t bzero32, int nfr, int n_cid, int n_fid,
+             int pkt_words, int *__restrict__ map, int *__restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool inval = false, range = false, acc = false;
     if (i < n) {
-        const uint32_t *h = pk + (size_t)i * PKT_WORDS;
+        const uint32_t *h = pk + (size_t)i * pkt_words;
         const uint32_t w0 = h[0], w4 = h[4], w5 = h[5];
         const uint32_t cid = w5 & 0xFFu, fid = (w5 >> 16) & 0xFFu;
         const uint32_t bc32 = (w5 >> 24) + ((w4 & 0xFFFFFFu) << 8);
         const int32_t rel = (int32_t)(bc32 - bzero32);
         if (w0 >> 31) {
             inval = true;
-        } else if (rel < 0 || rel >= nfr || fid >= 8u) {
+        } else if (rel < 0 || rel >= nfr || cid >= (uint32_t)n_cid || fid >= (uint32_t)n_fid) {
             range = true;
         } else {
             acc = true;
-            atomicMin(map + ((rel * 256 + (int)cid) * 8 + (int)fid), i);
+            atomicMin(map + ((rel * n_cid + (int)cid) * n_fid + (int)fid), i);
         }
     }
     // one atomic per wave per counter
@@ -164,6 +183,58 @@ k_reorder(const uint32_t *__restrict__ pk, const int *__restrict__ map, int nsna
     }
 }
 
+// Geometry 8 x 1024: workgroup = packet slot (cid, fid, frame).  Snapshot row
+// sn of the payload is 256 words; word w holds channels 4 w .. 4 w + 3 with
+// channel 4 w + j in byte 3 - j (the reference's order within a word), each
+// byte decoded as in k_reorder.  Thread t takes word t of the 8 rows and
+// writes the (im, re) pairs of its 4 channels for both polarisations as
+// 8-byte stores; a missing packet reads as 0xAA bytes (0 + 0j).  The slot of
+// fid 0 also writes the (frame, cid) bitmap, uint16 with bit fid set per
+// packet present, and adds the slots found to the unique count.
+__global__ void __launch_bounds__(256)
+k_reorder_8x1024(const uint32_t *__restrict__ pk, const int *__restrict__ map, int nsnap,
+                 int8_t *__restrict__ beng, uint16_t *__restrict__ bitmap, int *__restrict__ cnt) {
+    const int cid = blockIdx.x, fid = blockIdx.y, frame = blockIdx.z, t = threadIdx.x;
+    const int *slots = map + (frame * G2_CIDS + cid) * G2_FIDS;
+    const int p = slots[fid];
+    if (fid == 0 && t < 64) {
+        const unsigned long long b = __ballot(t < G2_FIDS && slots[t < G2_FIDS ? t : 0] != MAP_NONE);
+        if (t == 0) {
+            bitmap[frame * G2_CIDS + cid] = (uint16_t)b;
+            if (b) atomicAdd(cnt + CNT_UNIQUE, __popcll(b));
+        }
+    }
+    uint32_t words[G2_SNAPS];
+    if (p != MAP_NONE) {   // workgroup-uniform; no packet is read for a missing slot
+        const GAS uint32_t *src = gptr(pk) + (size_t)p * G2_PKT_WORDS + 8 + t;
+#pragma unroll
+        for (int sn = 0; sn < G2_SNAPS; ++sn) words[sn] = src[256 * sn];
+    } else {
+#pragma unroll
+        for (int sn = 0; sn < G2_SNAPS; ++sn) words[sn] = 0xAAAAAAAAu;
+    }
+#pragma unroll
+    for (int sn = 0; sn < G2_SNAPS; ++sn) {
+        const uint32_t word = words[sn];
+        uint32_t q0[2] = {0u, 0u}, q1[2] = {0u, 0u};   // pol 0, pol 1: channels 4t .. 4t + 3 as (im, re) bytes
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t b = (word >> (8 * (3 - j))) & 0xFFu;
+            const uint32_t im0 = (uint32_t)(uint8_t)(int8_t)((int)((b >> 4) & 3u) - 2);
+            const uint32_t re0 = (uint32_t)(uint8_t)(int8_t)((int)(b >> 6) - 2);
+            const uint32_t im1 = (uint32_t)(uint8_t)(int8_t)((int)(b & 3u) - 2);
+            const uint32_t re1 = (uint32_t)(uint8_t)(int8_t)((int)((b >> 2) & 3u) - 2);
+            q0[j >> 1] |= (im0 | (re0 << 8)) << (16 * (j & 1));
+            q1[j >> 1] |= (im1 | (re1 << 8)) << (16 * (j & 1));
+        }
+        const size_t row = (size_t)frame * 128 + (size_t)fid * G2_SNAPS + sn;
+        const size_t ch = (size_t)cid * (NCH / G2_CIDS) + 4 * t;
+        *reinterpret_cast<GAS uint64_t *>(gptr(beng) + (row * NCH + ch) * 2) = q0[0] | ((uint64_t)q0[1] << 32);
+        *reinterpret_cast<GAS uint64_t *>(gptr(beng) + (((size_t)nsnap + row) * NCH + ch) * 2) =
+            q1[0] | ((uint64_t)q1[1] << 32);
+    }
+}
+
 // ---------------------------------------------------------------- resampler
 
 // One block-pol of a call: where its spectra come from, where its frames go,
@@ -183,16 +254,6 @@ struct BpDesc {
 };
 static_assert(sizeof(BpDesc) == 64, "BpDesc layout");
 
-// Pointers read from a descriptor are generic to the compiler, which would
-// then emit flat memory instructions (counted on both the vector-memory and
-// LDS counters, so that every load waits for the previous ones).  These casts
-// state that they address global memory.
-#define GAS __attribute__((address_space(1)))
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-template <typename T>
-__device__ __forceinline__ GAS T *gptr(T *p) {
-    return (GAS T *)p;
-}
 
 // Word 3 of a raw buffer resource (untyped dword access, gfx9 family): loads
 // through it are range-checked against the resource's size and read 0 past it.
@@ -1199,7 +1260,9 @@ struct sdbe_handle {
     int snaps = NBLK;                // snapshots per block-pol
     int fpb = OUT_FRAMES_BLOCK;      // output frames per block-pol
     int nfr = NFR;                   // B-frames per group (DECIMATION_FACTOR of the rate)
-    int map_n = MAP_N;               // packet slots per group: nfr * 2048
+    int map_n = MAP_N;               // packet slots per group: nfr * packets per frame
+    int geom = SDBE_GEOM_REFERENCE;  // packet geometry
+    int n_cid = 256, n_fid = 8, pkt_words = PKT_WORDS;
     // re-framed group output (output-thread time alignment)
     int align_pending = 0;           // the next group starts an aligned stream
     int align_active = 0;
@@ -1587,7 +1650,8 @@ int unpack_map(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_
     LAUNCH(h, KID_FILL, 0, s, k_fill_i32, dim3(1), dim3(256), h->cnt, CNT_N, 0);
     if (n_pkts > 0)
         LAUNCH(h, KID_MAP, 1, s, k_packet_map, dim3((n_pkts + 255) / 256), dim3(256), (const uint32_t *)d_pkts,
-               n_pkts, (uint32_t)(b_zero & 0xFFFFFFFFull), h->nfr, h->map, h->cnt);
+               n_pkts, (uint32_t)(b_zero & 0xFFFFFFFFull), h->nfr, h->n_cid, h->n_fid, h->pkt_words, h->map,
+               h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
 }
@@ -1596,8 +1660,13 @@ int unpack_to_rows(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_
                    uint8_t *d_bitmap, hipStream_t s) {
     int rc = unpack_map(h, d_pkts, n_pkts, b_zero, s);
     if (rc) return rc;
-    LAUNCH(h, KID_REORDER, 1, s, k_reorder, dim3(128, h->nfr), dim3(256), (const uint32_t *)d_pkts,
-           (const int *)h->map, h->nfr * 128, h->beng, d_bitmap ? d_bitmap : h->bitmap, h->cnt);
+    if (h->geom == SDBE_GEOM_8X1024)
+        LAUNCH(h, KID_REORDER, 1, s, k_reorder_8x1024, dim3(G2_CIDS, G2_FIDS, h->nfr), dim3(256),
+               (const uint32_t *)d_pkts, (const int *)h->map, h->nfr * 128, h->beng,
+               (uint16_t *)(d_bitmap ? d_bitmap : h->bitmap), h->cnt);
+    else
+        LAUNCH(h, KID_REORDER, 1, s, k_reorder, dim3(128, h->nfr), dim3(256), (const uint32_t *)d_pkts,
+               (const int *)h->map, h->nfr * 128, h->beng, d_bitmap ? d_bitmap : h->bitmap, h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
 }
@@ -1711,6 +1780,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     if (cfg.max_spectra_blockpols < 1 || cfg.max_batch_blockpols < 0) return SDBE_ERR_ARG;
     if (cfg.c1_tiles_per_wg < 0 || cfg.c1_tiles_per_wg > 64) return SDBE_ERR_ARG;
     if (cfg.c1_interleave < 0 || cfg.c1_interleave > 64) return SDBE_ERR_ARG;
+    if (cfg.packet_geometry != SDBE_GEOM_REFERENCE && cfg.packet_geometry != SDBE_GEOM_8X1024) return SDBE_ERR_ARG;
     if (cfg.fft_precision < SDBE_FFT_FP32 || cfg.fft_precision > SDBE_FFT_FP16) return SDBE_ERR_ARG;
     trim::Geometry tg;
     if (cfg.rate_frac != SDBE_RATE_64_39) {
@@ -1733,8 +1803,14 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
         h->snaps = tg.D;
         h->fpb = tg.E;
         h->nfr = tg.D;
-        h->map_n = tg.D * 256 * 8;
     }
+    h->geom = cfg.packet_geometry;
+    if (h->geom == SDBE_GEOM_8X1024) {
+        h->n_cid = G2_CIDS;
+        h->n_fid = G2_FIDS;
+        h->pkt_words = G2_PKT_WORDS;
+    }
+    h->map_n = h->nfr * h->n_cid * h->n_fid;
     h->cap_bp = 2 * SDBE_BLOCKS_PER_GROUP;
     if (cfg.max_spectra_blockpols > h->cap_bp) h->cap_bp = cfg.max_spectra_blockpols;
     {
@@ -1894,7 +1970,8 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     hipStream_t s = (hipStream_t)stream;
     // 64/39: stage A reads the packets (8-byte words: the buffer must be 8-byte
     // aligned, as device allocations are); band-trimmed or otherwise: the int8 rows
-    const bool direct = h->trim.frac == 0 && n_pkts > 0 && (((uintptr_t)d_pkts) & 7) == 0;
+    const bool direct = h->trim.frac == 0 && h->geom == SDBE_GEOM_REFERENCE && n_pkts > 0 &&
+                        (((uintptr_t)d_pkts) & 7) == 0;
     int rc = direct ? unpack_bitmap(h, d_pkts, n_pkts, b_zero, d_bitmap, s)
                     : unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;

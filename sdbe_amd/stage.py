@@ -33,7 +33,7 @@ class SdbeStage:
     def __init__(self, policy=L.THRESH_REFERENCE, out_mode=L.OUT_PAYLOAD, chunk=256, device=0,
                  station="XX", bdc=0, rx=0, max_spectra=256, fixed=(1.0, 0.0), use_graph=False,
                  c1_tiles_per_wg=0, max_batch=0, fft_precision=L.FFT_FP32, rate_frac=L.RATE_64_39,
-                 skip_mhz=0, c1_interleave=0):
+                 skip_mhz=0, c1_interleave=0, packet_geometry=L.GEOM_REFERENCE):
         self.lib = L.load()
         cfg = L.Config()
         L.check(self.lib.sdbe_default_config(C.byref(cfg)), "sdbe_default_config")
@@ -53,6 +53,9 @@ class SdbeStage:
         cfg.rate_frac = rate_frac
         cfg.skip_mhz = skip_mhz
         cfg.c1_interleave = c1_interleave
+        cfg.packet_geometry = packet_geometry
+        # bytes per packet of the configured geometry
+        self.pkt_bytes = 4 * L.PKT_WORDS_8X1024 if packet_geometry == L.GEOM_8X1024 else L.PKT_BYTES
         self.cfg = cfg
         # geometry of a block-pol: snapshots in, samples and frames out
         # (include/sdbe_stage.h, SDBE_RATE_*)
@@ -139,7 +142,7 @@ class SdbeStage:
         if out is None:
             out = self.group_out()
         assert out.is_contiguous() and out.numel() >= 2 * self.frames_per_group * self.frame_bytes
-        assert d_pkts.numel() >= n_pkts * L.PKT_BYTES
+        assert d_pkts.numel() >= n_pkts * self.pkt_bytes
         stride = out.numel() // 2
         rc = self.lib.sdbe_process_group(self.h, _ptr(d_pkts), n_pkts, b_zero, _ptr(out), stride,
                                          _ptr(bitmap), _stream(stream))
@@ -150,8 +153,13 @@ class SdbeStage:
         dev = "cuda:%d" % self.device
         b0 = torch.empty((self.group_frames, 128, L.CHANNELS, 2), dtype=torch.int8, device=dev)
         b1 = torch.empty_like(b0)
-        bm = torch.empty((39, 256), dtype=torch.uint8, device=dev)
-        assert d_pkts.numel() >= n_pkts * L.PKT_BYTES
+        # fid bitmap: uint8 [frames][256] (reference geometry) or uint16
+        # [frames][16] (8 x 1024 geometry)
+        if self.cfg.packet_geometry == L.GEOM_8X1024:
+            bm = torch.empty((self.group_frames, 16), dtype=torch.int16, device=dev)
+        else:
+            bm = torch.empty((self.group_frames, 256), dtype=torch.uint8, device=dev)
+        assert d_pkts.numel() >= n_pkts * self.pkt_bytes
         rc = self.lib.sdbe_unpack_group_int8(self.h, _ptr(d_pkts), n_pkts, b_zero, _ptr(b0), _ptr(b1),
                                              _ptr(bm), _stream(stream))
         L.check(rc, "sdbe_unpack_group_int8", self.h)

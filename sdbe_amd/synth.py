@@ -130,6 +130,46 @@ def packetize(codes0, codes1, b_first, secs=1000, df0=0, ref_epoch=30,
     return pk
 
 
+PKT_WORDS_8X1024 = 2056
+
+
+def packetize_8x1024(codes0, codes1, b_first, secs=1000, df0=0, ref_epoch=30, station=0x536D):
+    """The "8 snapshots x 1024 channels" packet geometry (SDBE_GEOM_8X1024):
+    two polarisations of codes [n_frames*128][16384][2] -> packets
+    (n_frames*256, 2056) uint32 ordered (frame, cid, fid).  Packet (frame,
+    cid, fid) carries snapshots 8 fid .. 8 fid + 7 of its B-frame for channels
+    1024 cid .. 1024 cid + 1023: row sn of the payload is 256 words, word w
+    holding channels 4 w .. 4 w + 3 with channel 4 w + j in byte 3 - j, each
+    byte the reference's code layout (pol1 im, pol1 re, pol0 im, pol0 re from
+    the LSB).  Header as the reference's with cid, fid in word 5."""
+    n_snap = codes0.shape[0]
+    assert n_snap % N_SNAP_PER_FRAME == 0 and codes1.shape == codes0.shape
+    n_frames = n_snap // N_SNAP_PER_FRAME
+    byte = (codes1[..., 1] | (codes1[..., 0] << 2) | (codes0[..., 1] << 4)
+            | (codes0[..., 0] << 6)).astype(np.uint8)
+    # (frame, fid, sn, cid, word, channel offset j) -> byte order q = 3 - j
+    byte = byte.reshape(n_frames, 16, 8, 16, 256, 4)[..., ::-1]
+    pay = np.ascontiguousarray(byte.transpose(0, 3, 1, 2, 4, 5))          # (frame, cid, fid, sn, w, q)
+    pay = pay.reshape(n_frames * 256, 8192).view(np.uint32)
+    pk = np.zeros((n_frames * 256, PKT_WORDS_8X1024), dtype=np.uint32)
+    pk[:, 8:] = pay
+    fr = np.repeat(np.arange(n_frames, dtype=np.int64), 256)
+    idx = np.tile(np.arange(256, dtype=np.int64), n_frames)
+    cid = idx // 16
+    fid = idx % 16
+    b = b_first + fr
+    n = np.arange(n_frames * 256, dtype=np.int64)
+    pk[:, 0] = (secs + (df0 + n) // 125000) & 0x3FFFFFFF
+    pk[:, 1] = ((df0 + n) % 125000) | (ref_epoch << 24)
+    pk[:, 2] = 8224 // 8
+    pk[:, 3] = station | (1 << 26)
+    pk[:, 4] = (b >> 8) & 0xFFFFFFFF
+    pk[:, 5] = (cid | (fid << 16) | ((b & 0xFF) << 24)).astype(np.uint32)
+    pk[:, 6] = n & 0xFFFFFFFF
+    pk[:, 7] = 0
+    return pk
+
+
 def shuffle_and_drop(pkts, rng, drop_frac=0.001):
     """Fisher-Yates shuffle of the packet order, then drop round(frac*n)
     seed-chosen packets.  Returns (packets, indices of dropped originals)."""
@@ -144,11 +184,13 @@ def shuffle_and_drop(pkts, rng, drop_frac=0.001):
 
 
 def make_group_packets(seed, n_groups=2, drop_frac=0.001, shuffle=True, tones=None,
-                       b_first=1000, frames=39):
+                       b_first=1000, frames=39, geometry=0):
     """Config-2 style input: ``n_groups`` groups of ``frames`` B-frames (39 at
     the 6/11 rate; BENG_FRAMES_PER_GROUP = DECIMATION_FACTOR of the rate,
-    software/aphids/sample_rates.h:67), both pols.  Returns a list of
-    (packets, b_zero) per group and the per-pol codes."""
+    software/aphids/sample_rates.h:67), both pols, in the reference packet
+    geometry (geometry 0) or the 8 x 1024 one (geometry 1,
+    ``packetize_8x1024``).  Returns a list of (packets, b_zero) per group and
+    the per-pol codes."""
     rng = rng_for(seed)
     n_snap = frames * N_SNAP_PER_FRAME * n_groups
     kw = {} if tones is None else {"tones": tones}
@@ -158,8 +200,12 @@ def make_group_packets(seed, n_groups=2, drop_frac=0.001, shuffle=True, tones=No
     per = frames * N_SNAP_PER_FRAME
     for g in range(n_groups):
         b0 = b_first + frames * g
-        pk = packetize(c0[g * per:(g + 1) * per], c1[g * per:(g + 1) * per], b0,
-                       df0=g * frames * 2048)
+        if geometry == 1:
+            pk = packetize_8x1024(c0[g * per:(g + 1) * per], c1[g * per:(g + 1) * per], b0,
+                                  df0=g * frames * 256)
+        else:
+            pk = packetize(c0[g * per:(g + 1) * per], c1[g * per:(g + 1) * per], b0,
+                           df0=g * frames * 2048)
         if shuffle:
             pk, _ = shuffle_and_drop(pk, rng, drop_frac)
         out.append((pk, b0))

@@ -51,7 +51,8 @@ int main(void) {
   P(sdbe_stream_desc_t, n_blocks) P(sdbe_stream_desc_t, d_out) P(sdbe_stream_desc_t, state)
   P(sdbe_stream_desc_t, thresh_policy) P(sdbe_stream_desc_t, latch_slot) P(sdbe_stream_desc_t, station_id)
   P(sdbe_stream_desc_t, rx_sideband) P(sdbe_stream_desc_t, reserved)
-  P(sdbe_config_t, station_id) P(sdbe_config_t, rx_sideband) P(sdbe_config_t, reserved)
+  P(sdbe_config_t, station_id) P(sdbe_config_t, rx_sideband) P(sdbe_config_t, c1_interleave)
+  P(sdbe_config_t, packet_geometry) P(sdbe_config_t, skip_mhz)
   P(sdbe_stream_state_t, psn) P(sdbe_stats_t, latched_threshold) P(sdbe_kernel_time_t, total_ms)
   return 0;
 }
@@ -66,7 +67,8 @@ int main(void) {
     assert int(out["sdbe_kernel_time_t"]) == C.sizeof(L.KernelTime)
     assert int(out["sdbe_config_t.station_id"]) == L.Config.station_id.offset
     assert int(out["sdbe_config_t.rx_sideband"]) == L.Config.rx_sideband.offset
-    assert int(out["sdbe_config_t.reserved"]) == L.Config.reserved.offset
+    for f in ("c1_interleave", "packet_geometry", "skip_mhz"):
+        assert int(out["sdbe_config_t." + f]) == getattr(L.Config, f).offset, f
     assert int(out["sdbe_stream_state_t.psn"]) == L.StreamState.psn.offset
     assert int(out["sdbe_stats_t.latched_threshold"]) == L.Stats.latched_threshold.offset
     assert int(out["sdbe_kernel_time_t.total_ms"]) == L.KernelTime.total_ms.offset

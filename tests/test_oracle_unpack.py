@@ -144,3 +144,76 @@ def test_unpack_policies_duplicates_invalid_range():
     # the duplicate first: it wins
     r3 = O.vdif_to_beng(np.concatenate([dup, pk]), b0, n_frames=1)
     assert not np.array_equal(r3.beng, ref.beng)
+
+
+# ---------------------------------------------- the 8 x 1024 packet geometry
+def test_8x1024_unpack_equals_reference_geometry_on_the_same_codes():
+    """The same codes packed both ways unpack to the same int8 rows: the
+    8 x 1024 decode and scatter agree with the reference geometry's (itself
+    pinned above against the vdif.py / swarm.py route)."""
+    rng = np.random.default_rng(11)
+    n_frames, b0 = 2, 5000
+    c0 = rng.integers(0, 4, (n_frames * 128, 16384, 2), dtype=np.uint8)
+    c1 = rng.integers(0, 4, (n_frames * 128, 16384, 2), dtype=np.uint8)
+    ref = O.vdif_to_beng(S.packetize(c0, c1, b0), b0, n_frames=n_frames)
+    g2 = O.vdif_to_beng_8x1024(S.packetize_8x1024(c0, c1, b0), b0, n_frames=n_frames)
+    assert np.array_equal(ref.beng, g2.beng)
+    assert g2.n_unique == n_frames * 256 and g2.n_missing == 0
+    assert np.all(g2.bitmap == 0xFFFF)
+
+
+def test_8x1024_hand_built_packet():
+    """A packet built by hand from the format's definition (not by the
+    packetiser): byte q of snapshot row sn holds channel 1024 cid + (q & ~3) +
+    3 - (q & 3); bits from the LSB pol1 im, pol1 re, pol0 im, pol0 re; value =
+    code - 2."""
+    cid, fid, b0 = 9, 13, 7777
+    pk = np.zeros((1, O.G2_PKT_WORDS), dtype=np.uint32)
+    pk[0, 4] = b0 >> 8
+    pk[0, 5] = cid | (fid << 16) | ((b0 & 0xFF) << 24)
+    pay = np.full(8 * 1024, 0xAA, dtype=np.uint8)        # every field code 2: value 0
+    # row 2, word 5, byte 0 -> channel 4 * 5 + 3 = 23; fields im1=0, re1=1, im0=2, re0=3
+    pay[2 * 1024 + 4 * 5 + 0] = 0b11_10_01_00
+    # row 7, word 255, byte 3 -> channel 1020
+    pay[7 * 1024 + 4 * 255 + 3] = 0b00_01_10_11
+    pk[0, 8:] = pay.view(np.uint32)
+    r = O.vdif_to_beng_8x1024(pk, b0, n_frames=1)
+    ch = 1024 * cid
+    assert tuple(r.beng[0, 0, 8 * fid + 2, ch + 23]) == (0, 1)       # pol 0 (im, re)
+    assert tuple(r.beng[1, 0, 8 * fid + 2, ch + 23]) == (-2, -1)     # pol 1
+    assert tuple(r.beng[0, 0, 8 * fid + 7, ch + 1020]) == (-1, -2)
+    assert tuple(r.beng[1, 0, 8 * fid + 7, ch + 1020]) == (1, 0)
+    nz = np.argwhere(r.beng != 0)
+    assert len(nz) == 6                 # the 8 fields set above, two of them value 0
+    assert r.bitmap[0, cid] == 1 << fid and r.bitmap.sum() == 1 << fid
+
+
+def test_8x1024_drops_duplicates_invalid_range():
+    """Dropped slots stay zero with their bitmap bit clear; a duplicate loses
+    to the lower packet index; invalid and out-of-window packets are counted
+    and ignored."""
+    rng = np.random.default_rng(12)
+    n_frames, b0 = 2, 300
+    c0 = rng.integers(0, 4, (n_frames * 128, 16384, 2), dtype=np.uint8)
+    c1 = rng.integers(0, 4, (n_frames * 128, 16384, 2), dtype=np.uint8)
+    pk = S.packetize_8x1024(c0, c1, b0)
+    full = O.vdif_to_beng_8x1024(pk, b0, n_frames=n_frames)
+    drop = np.array([3, 77, 300, 511])
+    keep = np.setdiff1d(np.arange(pk.shape[0]), drop)
+    dup = pk[5].copy()
+    dup[8:] ^= np.uint32(0xFFFFFFFF)                     # same slot, other payload, higher index
+    inval = pk[6].copy()
+    inval[0] |= np.uint32(1 << 31)
+    late = pk[7].copy()
+    late[5] = (late[5] & 0x00FFFFFF) | (((b0 + n_frames) & 0xFF) << 24)   # frame n_frames: out of window
+    late[4] = (b0 + n_frames) >> 8
+    r = O.vdif_to_beng_8x1024(np.concatenate([pk[keep], dup[None], inval[None], late[None]]), b0,
+                              n_frames=n_frames)
+    assert r.n_invalid == 1 and r.n_out_of_range == 1 and r.n_duplicate == 1
+    assert r.n_unique == pk.shape[0] - drop.size and r.n_missing == drop.size
+    want = full.beng.copy()
+    for p in drop:
+        f, cid, fid = p // 256, (p % 256) // 16, p % 16
+        want[:, f, 8 * fid:8 * fid + 8, 1024 * cid:1024 * (cid + 1)] = 0
+        assert not (r.bitmap[f, cid] >> fid) & 1
+    assert np.array_equal(r.beng, want)

@@ -37,6 +37,7 @@ ALG_BYTES = {
     "k_stage_a": 39 * 16384 * 2 + 39 * 16384 * 8,        # int8 rows in, V out
     "k_stage_a_pk": 39 * 16384 // 2 + 39 * 16384 * 8,    # the block-pol's 4 bits per channel in, V (+ Zf row 0) out
     "k_group_bitmap": 79872 * 4 + 39 * 256,              # packet map in, fid bitmap out
+    "k_stage_a_pk8": 39 * 16384 // 2 + 39 * 16384 * 8,   # as k_stage_a_pk, from 8 x 1024 packets
     "k_stage_b": 2 * 38 * 16384 * 8,                     # V rows 1..38 in, packed R2C rows out
     "k_stage_c1": 39 * 16384 * 8 + (1 << 20) * 8,        # packed R2C in, first C2R pass out
     "k_stage_c2": (1 << 20) * 8 + (1 << 20),             # second pass in, 4-bit codes out
@@ -316,7 +317,8 @@ def main():
     if args.geometry == "8x1024":
         npk = ppf * frames
         alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4,
-                          "k_reorder": npk * pay + 2 * 128 * frames * 32768})
+                          "k_reorder_8x1024": npk * pay + 2 * 128 * frames * 32768,
+                          "k_group_bitmap_8x1024": npk * 4 + frames * 16 * 2})
     if args.rate:
         # band-trimmed kernels, per block of D snapshots: int8 rows -> series ->
         # 128 segments of P samples -> 2-bit payload
@@ -325,6 +327,7 @@ def main():
         npk = ppf * d           # packets of a group of D B-frames
         alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4,
                           "k_reorder": npk * pay + 2 * 128 * d * 32768,
+                          "k_reorder_8x1024": npk * pay + 2 * 128 * d * 32768,
                           "k_trim_c2r": d * 16384 * 2 + d * 32768 * 4,
                           "k_trim_seg": d * 32768 * 4 + 128 * p * 4,
                           "k_trim_pack": 128 * p * 4 + 128 * p // 4,

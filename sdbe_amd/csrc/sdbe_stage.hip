@@ -403,6 +403,97 @@ k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, flo
     a_store(V, Zf, bpl, k, x);
 }
 
+// Stage A from packets of the 8 x 1024 geometry (the int8 rows are not
+// materialised).  Group snapshot S = 39 blk + s sits in B-frame S >> 7, packet
+// fid (S & 127) >> 3, payload row S & 7; channel c in packet cid c >> 10, row
+// word (c & 1023) >> 2, byte 3 - (c & 3), decoded as in k_reorder_8x1024.  The
+// workgroup's channels ka are 256 consecutive ones: one aligned 64-word run of
+// a packet row per snapshot.  Their mirrors kb (N - k, or k - 1 flipped) lie in
+// 65 words, staged as the 68-word run around them that starts on a 4-word
+// boundary and may straddle two cids.  Payload rows start 16-byte aligned
+// (8224-byte packets, 32-byte header, 1024-byte rows), so with a 16-byte
+// aligned buffer the 39 x 132 words are staged in LDS with 16-byte coalesced
+// loads (one packet lookup per snapshot and cid first); then every thread
+// decodes its two channels from LDS.  Same workgroup mapping and arithmetic as
+// k_stage_a_pk.
+constexpr int AP8_QA = 16, AP8_QB = 17, AP8_Q = AP8_QA + AP8_QB;   // staged 4-word units per snapshot
+constexpr int AP8_W = 4 * AP8_Q;                                     // staged words per snapshot (132)
+
+__global__ void __launch_bounds__(256, 2)
+k_stage_a_pk8(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, float2 *__restrict__ Zf,
+              Tables tb, const uint32_t *__restrict__ pk, const int *__restrict__ map) {
+    __shared__ uint4 seg[NBLK * AP8_Q];      // [snapshot][unit]
+    __shared__ int pidx[NBLK * 3];           // [snapshot][run A cid, run B first cid, run B second cid]
+    const int L = xcd_contig((int)blockIdx.x, (NCH / 256) * nb);
+    const int bpl = L >> 6, idx = L & 63;
+    const int w = (idx & 1) ? 63 - (idx >> 1) : (idx >> 1);
+    const int t = threadIdx.x, k = w * 256 + t;
+    const int flip = bpd[bpl].flip, pol = bpd[bpl].pol;
+    const int S0 = NBLK * bpd[bpl].blk;
+    // first word (channel / 4) of each run: ka over [256 w, 256 w + 255] (flipped:
+    // N - 1 - that); kb over [N - 256 w - 255, N - 256 w] (flipped: [256 w - 1,
+    // 256 w + 254]), clamped into the row and aligned down to 4 words; k = 0
+    // pairs with the zero Nyquist bin and reads nothing
+    const int wa0 = (flip ? NCH - 256 * w - 256 : 256 * w) >> 2;
+    const int wb0 = min(max((flip ? 256 * w - 1 : NCH - 256 * w - 255) >> 2, 0), NCH / 4 - 65) & ~3;
+    const int ca = wa0 >> 8, cb0 = wb0 >> 8, cb1 = (wb0 + 4 * AP8_QB - 1) >> 8;
+    if (t < NBLK * 3) {
+        const int sn = t / 3, r = t - 3 * sn, S = S0 + sn;
+        const int cid = r == 0 ? ca : (r == 1 ? cb0 : cb1);
+        pidx[t] = map[((S >> 7) * G2_CIDS + cid) * G2_FIDS + ((S & 127) >> 3)];
+    }
+    __syncthreads();
+    for (int e = t; e < NBLK * AP8_Q; e += 256) {
+        const int sn = e / AP8_Q, q = e - sn * AP8_Q;
+        const int W = q < AP8_QA ? wa0 + 4 * q : wb0 + 4 * (q - AP8_QA);
+        const int p = pidx[3 * sn + (q < AP8_QA ? 0 : ((W >> 8) == cb0 ? 1 : 2))];
+        const int row = (S0 + sn) & 7;
+        uint4 v = make_uint4(0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu);
+        if (p != MAP_NONE)
+            v = *reinterpret_cast<const uint4 *>(pk + (size_t)p * G2_PKT_WORDS + 8 + 256 * row + (W & 255));
+        seg[e] = v;
+    }
+    __syncthreads();
+    const int km = (NCH - k) & (NCH - 1);
+    const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
+    const uint32_t *seg32 = reinterpret_cast<const uint32_t *>(seg);
+    // k = 0: kb unused
+    const int oa = (ka >> 2) - wa0, ob = 4 * AP8_QA + min(max((kb >> 2) - wb0, 0), 4 * AP8_QB - 1);
+    const int pbit = pol == 0 ? 4 : 0;
+    const int sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
+    const float kz = k == 0 ? 0.f : 1.f;
+    const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
+    // the codes' -2 offsets in sum and dif
+    const float2 cs = make_float2(-2.f - 2.f * kz, 0.f), cd = make_float2(-2.f + 2.f * kz, -4.f * kz);
+    float2 x[NBLK];
+#pragma unroll
+    for (int s = 0; s < NBLK; ++s) {
+        const uint32_t wa = seg32[AP8_W * s + oa], wb = seg32[AP8_W * s + ob];
+        const float Ia = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha, 2u);
+        const float Ra = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha + 2u, 2u);
+        const float Ib = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb, 2u);
+        const float Rb = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb + 2u, 2u);
+        x[s] = a_input(Ra, Ia, Rb, Ib, kz, cs, cd, T);
+    }
+    dft39<-1>(x);
+    a_store(V, Zf, bpl, k, x);
+}
+
+// The 8 x 1024 geometry's uint16 fid bitmap of every (frame, cid) and the
+// count of filled packet slots (what k_reorder_8x1024 records, for the
+// packet-reading stage A).  Workgroup = frame, thread = (cid, fid).
+__global__ void __launch_bounds__(256)
+k_group_bitmap_8x1024(const int *__restrict__ map, uint16_t *__restrict__ bitmap, int *__restrict__ cnt) {
+    const int frame = blockIdx.x, t = threadIdx.x;
+    const unsigned long long b = __ballot(map[frame * G2_CIDS * G2_FIDS + t] != MAP_NONE);
+    const int l = t & 63;
+    if ((l & 15) == 0) {
+        const uint16_t m = (uint16_t)(b >> l);
+        bitmap[frame * G2_CIDS + (t >> 4)] = m;
+        if (m) atomicAdd(cnt + CNT_UNIQUE, __popc((unsigned)m));
+    }
+}
+
 // The fid bitmap byte of every (frame, cid) of the group and the count of
 // filled packet slots (what k_reorder records, for the packet-reading stage A).
 __global__ void __launch_bounds__(256)
@@ -1198,12 +1289,14 @@ __global__ void __launch_bounds__(64) k_spectra_desc(SpecDescArgs a, int n_bp, B
 // ======================================================================= host
 enum KernelId {
     KID_FILL, KID_MAP, KID_REORDER, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
-    KID_PACK, KID_HEADERS, KID_TRIM_C2R, KID_TRIM_SEG, KID_TRIM_PACK, KID_A_PK, KID_BITMAP, KID_N
+    KID_PACK, KID_HEADERS, KID_TRIM_C2R, KID_TRIM_SEG, KID_TRIM_PACK, KID_A_PK, KID_BITMAP,
+    KID_REORDER8, KID_A_PK8, KID_BITMAP8, KID_N
 };
 static const char *const kKernelNames[KID_N] = {
     "k_fill_i32", "k_packet_map", "k_reorder", "k_stage_a", "k_stage_b",
     "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers", "k_trim_c2r", "k_trim_seg",
-    "k_trim_pack", "k_stage_a_pk", "k_group_bitmap"};
+    "k_trim_pack", "k_stage_a_pk", "k_group_bitmap", "k_reorder_8x1024", "k_stage_a_pk8",
+    "k_group_bitmap_8x1024"};
 
 struct ProfRec {
     int kid;
@@ -1530,7 +1623,10 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
-        if (pk)
+        if (pk && h->geom == SDBE_GEOM_8X1024)
+            LAUNCH(h, KID_A_PK8, nb, s, k_stage_a_pk8, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->Zf, h->tb,
+                   pk, (const int *)h->map);
+        else if (pk)
             LAUNCH(h, KID_A_PK, nb, s, k_stage_a_pk, dim3((NCH / 256) * nb), dim3(256), bpd, nb, h->V, h->Zf, h->tb, pk,
                    (const int *)h->map);
         else
@@ -1661,7 +1757,7 @@ int unpack_to_rows(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_
     int rc = unpack_map(h, d_pkts, n_pkts, b_zero, s);
     if (rc) return rc;
     if (h->geom == SDBE_GEOM_8X1024)
-        LAUNCH(h, KID_REORDER, 1, s, k_reorder_8x1024, dim3(G2_CIDS, G2_FIDS, h->nfr), dim3(256),
+        LAUNCH(h, KID_REORDER8, 1, s, k_reorder_8x1024, dim3(G2_CIDS, G2_FIDS, h->nfr), dim3(256),
                (const uint32_t *)d_pkts, (const int *)h->map, h->nfr * 128, h->beng,
                (uint16_t *)(d_bitmap ? d_bitmap : h->bitmap), h->cnt);
     else
@@ -1675,8 +1771,12 @@ int unpack_bitmap(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t
                   uint8_t *d_bitmap, hipStream_t s) {
     int rc = unpack_map(h, d_pkts, n_pkts, b_zero, s);
     if (rc) return rc;
-    LAUNCH(h, KID_BITMAP, 1, s, k_group_bitmap, dim3(h->nfr), dim3(256), (const int *)h->map,
-           d_bitmap ? d_bitmap : h->bitmap, h->cnt);
+    if (h->geom == SDBE_GEOM_8X1024)
+        LAUNCH(h, KID_BITMAP8, 1, s, k_group_bitmap_8x1024, dim3(h->nfr), dim3(256), (const int *)h->map,
+               (uint16_t *)(d_bitmap ? d_bitmap : h->bitmap), h->cnt);
+    else
+        LAUNCH(h, KID_BITMAP, 1, s, k_group_bitmap, dim3(h->nfr), dim3(256), (const int *)h->map,
+               d_bitmap ? d_bitmap : h->bitmap, h->cnt);
     HIP_TRY(h, hipGetLastError());
     return SDBE_OK;
 }
@@ -1968,10 +2068,11 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     if (out_pol_stride_bytes < (int64_t)blocks * h->fpb * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    // 64/39: stage A reads the packets (8-byte words: the buffer must be 8-byte
-    // aligned, as device allocations are); band-trimmed or otherwise: the int8 rows
-    const bool direct = h->trim.frac == 0 && h->geom == SDBE_GEOM_REFERENCE && n_pkts > 0 &&
-                        (((uintptr_t)d_pkts) & 7) == 0;
+    // 64/39: stage A reads the packets (8-byte words for the reference geometry,
+    // 16-byte units for 8 x 1024: the buffer must be aligned to them, as device
+    // allocations are); band-trimmed or otherwise: the int8 rows
+    const uintptr_t pk_align = h->geom == SDBE_GEOM_8X1024 ? 15 : 7;
+    const bool direct = h->trim.frac == 0 && n_pkts > 0 && (((uintptr_t)d_pkts) & pk_align) == 0;
     int rc = direct ? unpack_bitmap(h, d_pkts, n_pkts, b_zero, d_bitmap, s)
                     : unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;

@@ -8,7 +8,9 @@ reference packs 128 snapshots x 8 channels).  Bars:
 * with no packet dropped, sdbe_process_group's output equals, byte for byte,
   the same codes sent in the reference geometry;
 * with drops, each block-pol's payload equals sdbe_process_spectra on the
-  oracle's rows of that block-pol (PER_BLOCK thresholds).
+  oracle's rows of that block-pol (PER_BLOCK thresholds), and the fid bitmap
+  and counters of sdbe_process_group (whose stage A reads the packets) equal
+  the oracle's.
 """
 import sys
 
@@ -100,9 +102,15 @@ def test_process_group_8x1024_with_drops_equals_spectra_path(torch_cuda, codes):
     pk, dropped = S.shuffle_and_drop(S.packetize_8x1024(c0, c1, B0), rng, drop_frac=0.01)
     g2 = SdbeStage(policy=L.THRESH_PER_BLOCK, packet_geometry=L.GEOM_8X1024)
     _say("[g2-drops] gpu process_group (%d of %d packets dropped)" % (dropped.size, dropped.size + pk.shape[0]))
-    out = g2.process_group(g2.to_device_packets(pk), pk.shape[0], B0).cpu().numpy()
+    bm = torch.full((39, 16), -1, dtype=torch.int16, device="cuda")
+    out = g2.process_group(g2.to_device_packets(pk), pk.shape[0], B0, bitmap=bm).cpu().numpy()
+    got = g2.stats()
     g2.close()
-    rows = O.vdif_to_beng_8x1024(pk, B0).beng.reshape(2, 128, 39, 16384, 2)
+    ref = O.vdif_to_beng_8x1024(pk, B0)
+    # the packet-reading stage A records the fid bitmap and the counters as the unpack does
+    assert np.array_equal(bm.cpu().numpy().view(np.uint16), ref.bitmap)
+    assert got["n_unique"] == ref.n_unique and got["n_missing"] == ref.n_missing
+    rows = ref.beng.reshape(2, 128, 39, 16384, 2)
     blocks = (0, 1, 17, 63, 64, 100, 127)
     spec = SdbeStage(policy=L.THRESH_PER_BLOCK, max_spectra=2 * len(blocks))
     d = torch.from_numpy(np.ascontiguousarray(

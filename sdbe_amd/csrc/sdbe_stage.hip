@@ -339,15 +339,21 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, float2
 // 0xAA bytes, which decode to 0 + 0j (as in k_reorder).
 // The workgroup's 256 channels k lie in 32 channel groups and their mirrors
 // (or, flipped, their reversals) in 33: the 39 snapshot word pairs of those 65
-// groups are staged in LDS with coalesced loads (the packet indices first),
-// then every thread decodes its two channels from LDS.  Same workgroup mapping
-// and arithmetic as k_stage_a.
+// groups are staged in LDS with coalesced 16-byte loads (the packet indices
+// first), then every thread decodes its two channels from LDS.  A group's
+// pairs come from rows r0 = S0 - 128 f0 .. 127 of its frame-f0 packet, then
+// rows 0 .. of its frame-f1 packet; with snapshot s at LDS word 2 s + 2 (r0 & 1)
+// both runs start on 16-byte boundaries (1056-byte packets, 32-byte header),
+// so 20 units of 4 words cover a group.  Same workgroup mapping and arithmetic
+// as k_stage_a.
 constexpr int APK_GA = 32, APK_GB = 33, APK_G = APK_GA + APK_GB;
+constexpr int APK_U = 20;        // 16-byte units loaded per group
+constexpr int APK_US = 21;       // units per group in LDS (84 words: conflict-free decode reads)
 
 __global__ void __launch_bounds__(256, 2)
 k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb,
              const uint32_t *__restrict__ pk, const int *__restrict__ map) {
-    __shared__ uint2 seg[APK_G * NBLK];     // [group slot][snapshot] word pairs
+    __shared__ uint4 seg[APK_G * APK_US];   // [group slot][unit]: snapshot s at word 2 s + off
     __shared__ int pidx[APK_G * 2];         // [group slot][frame f0, f1] packet index
     const int L = xcd_contig((int)blockIdx.x, (NCH / 256) * nb);
     const int bpl = L >> 6, idx = L & 63;
@@ -356,6 +362,9 @@ k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, flo
     const int flip = bpd[bpl].flip, pol = bpd[bpl].pol;
     const int S0 = NBLK * bpd[bpl].blk, f0 = S0 >> 7, s_sw = 128 * (f0 + 1) - S0;
     const int f1 = min(f0 + 1, NFR - 1);
+    // LDS word offset of snapshot 0; first payload word of the f0 run (16-byte
+    // aligned); LDS word where the f1 run starts (past the group when s_sw >= 39)
+    const int r0 = S0 - 128 * f0, off = 2 * (r0 & 1), a0 = 2 * r0 - off, E = 256 - a0;
     // first group of each range: k (32 groups), mirror / reversed mirror (33)
     const int ga0 = flip ? 2047 - 32 * w - 31 : 32 * w;
     const int gb0 = flip ? 32 * w - 1 : 2048 - 32 * w - 32;
@@ -366,14 +375,14 @@ k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, flo
         pidx[t] = map[(fr * 256 + cid) * 8 + fid];
     }
     __syncthreads();
-    for (int e = t; e < APK_G * NBLK; e += 256) {
-        const int i = e / NBLK, sn = e - i * NBLK;
-        const bool first = sn < s_sw;
+    for (int e = t; e < APK_G * APK_U; e += 256) {
+        const int i = e / APK_U, j = 4 * (e - i * APK_U);
+        const bool first = j < E;
         const int p = pidx[2 * i + (first ? 0 : 1)];
-        const int row = first ? S0 - 128 * f0 + sn : sn - s_sw;
-        uint2 v = make_uint2(0xAAAAAAAAu, 0xAAAAAAAAu);
-        if (p != MAP_NONE) v = *reinterpret_cast<const uint2 *>(pk + (size_t)p * PKT_WORDS + 8 + 2 * row);
-        seg[e] = v;
+        const int word = first ? a0 + j : j - E;
+        uint4 v = make_uint4(0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu);
+        if (p != MAP_NONE) v = *reinterpret_cast<const uint4 *>(pk + (size_t)p * PKT_WORDS + 8 + word);
+        seg[i * APK_US + (j >> 2)] = v;
     }
     __syncthreads();
     const int km = (NCH - k) & (NCH - 1);      // k = 0 pairs with the zero Nyquist bin
@@ -381,7 +390,8 @@ k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, flo
     // word offsets into seg (as uint32), and the nibble's bit
     const uint32_t *seg32 = reinterpret_cast<const uint32_t *>(seg);
     const int sa = (ka >> 3) - ga0, sb = min(max((kb >> 3) - gb0, 0), APK_GB - 1);   // k = 0: kb unused
-    const int oa = 2 * NBLK * sa + ((ka & 7) >> 2), ob = 2 * NBLK * (APK_GA + sb) + ((kb & 7) >> 2);
+    const int oa = 4 * APK_US * sa + off + ((ka & 7) >> 2);
+    const int ob = 4 * APK_US * (APK_GA + sb) + off + ((kb & 7) >> 2);
     const int pbit = pol == 0 ? 4 : 0;
     const int sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
     const float kz = k == 0 ? 0.f : 1.f;
@@ -2068,11 +2078,10 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     if (out_pol_stride_bytes < (int64_t)blocks * h->fpb * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    // 64/39: stage A reads the packets (8-byte words for the reference geometry,
-    // 16-byte units for 8 x 1024: the buffer must be aligned to them, as device
-    // allocations are); band-trimmed or otherwise: the int8 rows
-    const uintptr_t pk_align = h->geom == SDBE_GEOM_8X1024 ? 15 : 7;
-    const bool direct = h->trim.frac == 0 && n_pkts > 0 && (((uintptr_t)d_pkts) & pk_align) == 0;
+    // 64/39: stage A reads the packets in 16-byte units (the buffer must be
+    // 16-byte aligned, as device allocations are); band-trimmed or otherwise:
+    // the int8 rows
+    const bool direct = h->trim.frac == 0 && n_pkts > 0 && (((uintptr_t)d_pkts) & 15) == 0;
     int rc = direct ? unpack_bitmap(h, d_pkts, n_pkts, b_zero, d_bitmap, s)
                     : unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;

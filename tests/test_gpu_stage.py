@@ -400,19 +400,20 @@ def test_process_group_missing_frames_and_empty(torch_cuda, one_group):
     ref.close()
 
 
-def test_process_group_unaligned_packets(torch_cuda, one_group):
-    """A packet buffer that is only 4-byte aligned takes the int8-row path
-    (stage A's packet reads are 8-byte words); the output is the same."""
+@pytest.mark.parametrize("shift", [4, 8])
+def test_process_group_unaligned_packets(torch_cuda, one_group, shift):
+    """A packet buffer that is not 16-byte aligned takes the int8-row path
+    (stage A's packet reads are 16-byte units); the output is the same."""
     torch = torch_cuda
     from sdbe_amd import _lib as L
     _, pk_shuf, b0 = one_group
     st = _stage(policy=L.THRESH_PER_BLOCK, out_mode=L.OUT_PAYLOAD)
     d = st.to_device_packets(pk_shuf)
-    raw = torch.empty(d.numel() + 4, dtype=torch.uint8, device="cuda")
-    raw[4:].copy_(d)
-    du = raw[4:]
-    assert du.data_ptr() % 8 == 4
-    _say("[group-align] gpu process_group aligned / offset by 4 bytes")
+    raw = torch.empty(d.numel() + shift, dtype=torch.uint8, device="cuda")
+    raw[shift:].copy_(d)
+    du = raw[shift:]
+    assert du.data_ptr() % 16 == shift
+    _say("[group-align] gpu process_group aligned / offset by %d bytes" % shift)
     want = st.process_group(d, pk_shuf.shape[0], b0).cpu().numpy()
     got = st.process_group(du, pk_shuf.shape[0], b0).cpu().numpy()
     assert np.array_equal(got, want)

@@ -329,6 +329,9 @@ def main():
                           "k_reorder": npk * pay + 2 * 128 * d * 32768,
                           "k_reorder_8x1024": npk * pay + 2 * 128 * d * 32768,
                           "k_trim_c2r": d * 16384 * 2 + d * 32768 * 4,
+                          "k_trim_c2r_pk": d * 16384 // 2 + d * 32768 * 4,
+                          "k_group_bitmap": npk * 4 + d * 256,
+                          "k_group_bitmap_8x1024": npk * 4 + d * 16 * 2,
                           "k_trim_seg": d * 32768 * 4 + 128 * p * 4,
                           "k_trim_pack": 128 * p * 4 + 128 * p // 4,
                           "k_headers": e * 32, "k_stats": 128 * 16})

@@ -1300,13 +1300,13 @@ __global__ void __launch_bounds__(64) k_spectra_desc(SpecDescArgs a, int n_bp, B
 enum KernelId {
     KID_FILL, KID_MAP, KID_REORDER, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
     KID_PACK, KID_HEADERS, KID_TRIM_C2R, KID_TRIM_SEG, KID_TRIM_PACK, KID_A_PK, KID_BITMAP,
-    KID_REORDER8, KID_A_PK8, KID_BITMAP8, KID_N
+    KID_REORDER8, KID_A_PK8, KID_BITMAP8, KID_TRIM_C2R_PK, KID_N
 };
 static const char *const kKernelNames[KID_N] = {
     "k_fill_i32", "k_packet_map", "k_reorder", "k_stage_a", "k_stage_b",
     "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers", "k_trim_c2r", "k_trim_seg",
     "k_trim_pack", "k_stage_a_pk", "k_group_bitmap", "k_reorder_8x1024", "k_stage_a_pk8",
-    "k_group_bitmap_8x1024"};
+    "k_group_bitmap_8x1024", "k_trim_c2r_pk"};
 
 struct ProfRec {
     int kid;
@@ -1588,15 +1588,20 @@ void launch_c2(sdbe_handle_t *h, int nb, hipStream_t s, const BpDesc *bpd) {
 // threshold policy and the natural-order packing.
 template <int D, int E>
 int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32, double *d_stats,
-             hipStream_t s) {
+             hipStream_t s, const uint32_t *pk) {
     constexpr int P = 256 * E;
     const auto kc2r = k_trim_c2r<D>;
+    const auto kc2r_pk = h->geom == SDBE_GEOM_8X1024 ? k_trim_c2r_pk<D, true> : k_trim_c2r_pk<D, false>;
     const auto kseg_f = k_trim_seg<D, E, true>;
     const auto kseg_c = k_trim_seg<D, E, false>;
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
-        LAUNCH(h, KID_TRIM_C2R, nb, s, kc2r, dim3(D * nb), dim3(1024), bpd, h->V, h->tb);
+        if (pk)
+            LAUNCH(h, KID_TRIM_C2R_PK, nb, s, kc2r_pk, dim3(D * nb), dim3(1024), bpd, h->V, h->tb, pk,
+                   (const int *)h->map);
+        else
+            LAUNCH(h, KID_TRIM_C2R, nb, s, kc2r, dim3(D * nb), dim3(1024), bpd, h->V, h->tb);
         if (f32)
             LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_f, dim3(trim::NSEG * nb), dim3(trim::seg_threads(D)),
                    (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip);
@@ -1622,14 +1627,15 @@ int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od,
 
 // Run stages A..C2 (+ pack) over the n_bp block-pols described by the device
 // table d_bpd, in chunks.  f32: write float samples instead of codes.
-// pk: the group's packets when stage A reads them directly (the block-pols'
-// blk / pol then locate their snapshots through the packet map), else null.
+// pk: the group's packets when the first kernel (stage A, or the band-trimmed
+// per-snapshot C2R) reads them directly (the block-pols' blk / pol then locate
+// their snapshots through the packet map), else null.
 int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32,
                   double *d_stats, hipStream_t s, const uint32_t *pk = nullptr) {
-    if (h->trim.frac == 6) return run_trim<39, 32>(h, d_bpd, n_bp, od, f32, d_stats, s);
-    if (h->trim.frac == 8) return run_trim<13, 8>(h, d_bpd, n_bp, od, f32, d_stats, s);
-    if (h->trim.frac == 10) return run_trim<65, 64>(h, d_bpd, n_bp, od, f32, d_stats, s);
-    if (h->trim.frac == 11) return run_trim<143, 128>(h, d_bpd, n_bp, od, f32, d_stats, s);
+    if (h->trim.frac == 6) return run_trim<39, 32>(h, d_bpd, n_bp, od, f32, d_stats, s, pk);
+    if (h->trim.frac == 8) return run_trim<13, 8>(h, d_bpd, n_bp, od, f32, d_stats, s, pk);
+    if (h->trim.frac == 10) return run_trim<65, 64>(h, d_bpd, n_bp, od, f32, d_stats, s, pk);
+    if (h->trim.frac == 11) return run_trim<143, 128>(h, d_bpd, n_bp, od, f32, d_stats, s, pk);
     for (int bp0 = 0; bp0 < n_bp; bp0 += h->chunk) {
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
@@ -2078,10 +2084,10 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
     if (out_pol_stride_bytes < (int64_t)blocks * h->fpb * od.frame_bytes) return SDBE_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     hipStream_t s = (hipStream_t)stream;
-    // 64/39: stage A reads the packets in 16-byte units (the buffer must be
-    // 16-byte aligned, as device allocations are); band-trimmed or otherwise:
-    // the int8 rows
-    const bool direct = h->trim.frac == 0 && n_pkts > 0 && (((uintptr_t)d_pkts) & 15) == 0;
+    // The first kernel (stage A at 64/39, the per-snapshot C2R band-trimmed)
+    // reads the packets in 8- or 16-byte units: with a 16-byte aligned buffer,
+    // as device allocations are; otherwise the int8 rows
+    const bool direct = n_pkts > 0 && (((uintptr_t)d_pkts) & 15) == 0;
     int rc = direct ? unpack_bitmap(h, d_pkts, n_pkts, b_zero, d_bitmap, s)
                     : unpack_to_rows(h, d_pkts, n_pkts, b_zero, d_bitmap, s);
     if (rc) return rc;

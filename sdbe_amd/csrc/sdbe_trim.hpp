@@ -286,6 +286,78 @@ k_trim_c2r(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb) {
     for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
 }
 
+// SwarmC2R of one snapshot read from the group's packets (the int8 rows are
+// not materialised): the same arithmetic as k_trim_c2r.  Snapshot S = D blk + s
+// of the group sits in B-frame S >> 7.  Its 16384 channels' payload bytes are
+// staged in LDS (aliasing the FFT work array, before the FFT uses it) so that
+// channel c is byte 3 - (c & 3) of staged word c >> 2 in both geometries:
+// * reference (G2 false): channel group g = c >> 3 is packet (cid 2 (g >> 4) +
+//   (g & 1), fid (g >> 1) & 7), payload words 2 r, 2 r + 1 for row r = S & 127
+//   (vdif_inout_gpu_thread.cu:442, 464-494): one 8-byte load per group;
+// * 8 x 1024 (G2 true): packet (cid c >> 10, fid (S & 127) >> 3), payload row
+//   S & 7, word (c & 1023) >> 2: one 16-byte load per thread.
+// Both polarisations' codes share a byte (pol 0 in bits 4-7: re 6-7, im 4-5;
+// pol 1 in bits 0-3); a missing packet is staged as 0xAA bytes, which decode to
+// 0 + 0j.  Workgroups are taken in XCD-contiguous order, so the snapshots of a
+// block, whose words share cache lines, are read through one XCD's L2.
+template <int D, bool G2>
+__global__ void __launch_bounds__(1024)
+k_trim_c2r_pk(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb, const uint32_t *__restrict__ pk,
+              const int *__restrict__ map) {
+    __shared__ float lds[LDS_FFT_N];
+    __shared__ FftTw ftw;
+    fft_tw_load(&ftw, tb.fft);
+    const int L = xcd_contig((int)blockIdx.x, (int)gridDim.x);
+    const int bpl = L / D, s = L - bpl * D, t = threadIdx.x;
+    const int flip = bpd[bpl].flip, pol = bpd[bpl].pol;
+    const int S = D * bpd[bpl].blk + s, frame = S >> 7;
+    uint32_t *stg = reinterpret_cast<uint32_t *>(lds);   // NCH / 4 words
+    if constexpr (G2) {
+        const int cid = t >> 6;
+        const int p = map[(frame * G2_CIDS + cid) * G2_FIDS + ((S & 127) >> 3)];
+        uint4 v = make_uint4(0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu);
+        if (p != MAP_NONE)
+            v = *reinterpret_cast<const uint4 *>(pk + (size_t)p * G2_PKT_WORDS + 8 + 256 * (S & 7) + 4 * (t & 63));
+        reinterpret_cast<uint4 *>(stg)[t] = v;
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int g = t + 1024 * h;
+            const int cid = 2 * (g >> 4) + (g & 1), fid = (g >> 1) & 7;
+            const int p = map[(frame * 256 + cid) * 8 + fid];
+            uint2 v = make_uint2(0xAAAAAAAAu, 0xAAAAAAAAu);
+            if (p != MAP_NONE) v = *reinterpret_cast<const uint2 *>(pk + (size_t)p * PKT_WORDS + 8 + 2 * (S & 127));
+            reinterpret_cast<uint2 *>(stg)[g] = v;
+        }
+    }
+    __syncthreads();
+    const int pbit = pol == 0 ? 4 : 0;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = t + 1024 * q;
+        const int km = (NCH - k) & (NCH - 1);
+        const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
+        const float kz = k == 0 ? 0.f : 1.f;
+        const uint32_t wa = stg[ka >> 2], wb = stg[kb >> 2];
+        const uint32_t sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
+        // codes: im in the nibble's bits 0-1, re in bits 2-3; value = code - 2
+        const float ra = (float)((int)__builtin_amdgcn_ubfe(wa, sha + 2u, 2u) - 2);
+        const float ia = (float)((int)__builtin_amdgcn_ubfe(wa, sha, 2u) - 2);
+        const float rb = (float)((int)__builtin_amdgcn_ubfe(wb, shb + 2u, 2u) - 2);
+        const float ib = (float)((int)__builtin_amdgcn_ubfe(wb, shb, 2u) - 2);
+        const float2 a = make_float2(ra, kz * ia);
+        const float2 b = make_float2(kz * rb, kz * ib);
+        const float2 sum = cadd(a, cconj(b)), dif = csub(a, cconj(b));
+        v[q] = cadd(sum, mul_si<+1>(cmul(tw<+1>(tb.n32k, (uint32_t)k), dif)));
+    }
+    __syncthreads();   // the staged words are read; the FFT reuses the array
+    fft16k_t<+1>(v, lds, &ftw);   // element fft16k_tord(t) + 1024 q in v[q]
+    float2 *dst = T + ((size_t)bpl * D + s) * NCH + fft16k_tord(t);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
+}
+
 // SwarmR2C + R2dbeC2R of segment g of block bpl (workgroup = segment,
 // trim::seg_threads(D) threads).  Output: P samples (numpy units) at
 // out[bpl] + g P, where out is bpd[bpl].out (float samples) when F32, else

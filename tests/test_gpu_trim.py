@@ -11,7 +11,9 @@ Bars, as for the FP32 64/39 pipeline:
   one per block-pol between); headers bit-exact (frames of 8224 bytes, E per
   block-pol);
 * a whole group through sdbe_process_group (shuffled packets with drops,
-  REFERENCE latch): the same code bar on blocks of both polarisations.
+  REFERENCE latch): the same code bar on blocks of both polarisations;
+* the group's first kernel reading the packets (either geometry) gives the
+  same bytes, bitmap and counters as reading the unpacked int8 rows.
 Rejected: unknown presets, a skip past the band, precision variants, ragged
 batches and re-framed output in a band-trimmed handle.
 """
@@ -159,6 +161,43 @@ def test_trim_group_reference_policy(torch_cuda, frac):
     _say(tally.report())
     tally.check()
     st.close()
+
+
+@pytest.mark.parametrize("geometry", [0, 1])
+@pytest.mark.parametrize("frac", [6, 8, 10, 11])
+def test_trim_group_packet_c2r_matches_rows(torch_cuda, frac, geometry):
+    """From a 16-byte aligned buffer the band-trimmed group's first kernel reads
+    the packets (k_trim_c2r_pk); from a buffer 8 bytes off it reads the int8
+    rows the unpack writes.  Whole group, both polarisations, every block: the
+    output bytes, the fid bitmap and the counters are identical."""
+    torch = torch_cuda
+    from sdbe_amd import _lib as L
+    D, E, R, P, skip = O.trim_geometry(frac, SKIP_A[frac])
+    _say("[trim-pk] %d/11 geometry %d: generating a group of %d B-frames" % (frac, geometry, D))
+    groups, _ = S.make_group_packets(seed=830 + frac + 10 * geometry, n_groups=1, drop_frac=0.002, frames=D,
+                                     geometry=geometry)
+    pk, b0 = groups[0]
+    del groups
+    gk = L.GEOM_8X1024 if geometry else L.GEOM_REFERENCE
+    st = _stage(frac, chunk=256, packet_geometry=gk)
+    d = st.to_device_packets(pk)
+    raw = torch.empty(d.numel() + 8, dtype=torch.uint8, device="cuda")
+    raw[8:].copy_(d)
+    du = raw[8:]
+    assert d.data_ptr() % 16 == 0 and du.data_ptr() % 16 == 8
+    bshape, bdt = ((D, 16), torch.int16) if geometry else ((D, 256), torch.uint8)
+    bm_a = torch.full(bshape, -1, dtype=bdt, device="cuda")
+    bm_b = torch.full(bshape, -1, dtype=bdt, device="cuda")
+    _say("[trim-pk] %d/11 geometry %d: process_group from packets (%d)" % (frac, geometry, pk.shape[0]))
+    a = st.process_group(d, pk.shape[0], b0, bitmap=bm_a).cpu().numpy()
+    sa = st.stats()
+    _say("[trim-pk] %d/11 geometry %d: process_group from int8 rows" % (frac, geometry))
+    b = st.process_group(du, pk.shape[0], b0, bitmap=bm_b).cpu().numpy()
+    sb = st.stats()
+    st.close()
+    assert np.array_equal(a, b)
+    assert torch.equal(bm_a, bm_b)
+    assert sa["n_unique"] == sb["n_unique"] and sa["n_missing"] == sb["n_missing"] and sa["n_unique"] > 0
 
 
 def test_trim_rejects(torch_cuda):

@@ -40,8 +40,8 @@ ALG_BYTES = {
     "k_stage_a_pk8": 39 * 16384 // 2 + 39 * 16384 * 8,   # as k_stage_a_pk, from 8 x 1024 packets
     "k_stage_b": 2 * 38 * 16384 * 8,                     # V rows 1..38 in, packed R2C rows out
     "k_stage_c1": 39 * 16384 * 8 + (1 << 20) * 8,        # packed R2C in, first C2R pass out
-    "k_stage_c2": (1 << 20) * 8 + (1 << 20),             # second pass in, 4-bit codes out
-    "k_pack": (1 << 20) + 64 * 8192,                     # codes in, payloads out
+    "k_stage_c2": (1 << 20) * 8 + (1 << 19),             # second pass in, 2-bit codes out (4 per byte)
+    "k_pack": (1 << 19) + 64 * 8192,                     # codes in, payloads out
     "k_headers": 64 * 32,
     "k_stats": 513 * 16,
     "k_fill_i32": 79872 * 4,
@@ -366,9 +366,25 @@ def main():
     torch.cuda.synchronize()
     wprof = prof_merge()
     wsteps = max(args.warmup, 1)
-    dname = max(wprof.items(), key=lambda kv: kv[1][1])[0] if wprof else None
+    # Events around every launch stretch the persistent kernels more than the
+    # others, so the warmup table only nominates: the three longest kernels are
+    # each timed alone (events around that kernel's launches only) over two more
+    # untimed steps, and the one with the most time per step is the roofline kernel.
+    cands = {}
+    for name, _ in sorted(wprof.items(), key=lambda kv: -kv[1][1])[:3]:
+        for h in sts:
+            h.profile_select(name)
+            h.profile_read()
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        p = prof_merge()
+        if name in p:
+            cands[name] = p[name]
+    dname = max(cands.items(), key=lambda kv: kv[1][1])[0] if cands else None
     for h in sts:
         h.profile_select(dname)
+        h.profile_read()
     log("[bench] timing %d steps (events around %s only)" % (args.steps, dname))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -403,6 +419,13 @@ def main():
     d_bytes_per_launch = alg_bytes[dname] * du / dl
     d_avg_s = dms * 1e-3 / dl
     achieved = d_bytes_per_launch / d_avg_s / 1e9
+    roof_cands = {}
+    for name, (cl, cms, cu) in cands.items():
+        ab = alg_bytes.get(name)
+        a = ab * cu / (cms * 1e-3) / 1e9 if ab and cms > 0 else None
+        roof_cands[name] = {"avg_us": round(1e3 * cms / cl, 1), "ms_per_step": round(cms / 2, 4),
+                            "achieved_GBps": round(a, 1) if a else None,
+                            "frac": round(a * 1e9 / HBM_PEAK, 4) if a else None}
 
     rec = {
         "metric": METRIC,
@@ -434,7 +457,8 @@ def main():
             "c1_tiles_per_wg": args.c1_tiles or 8,
             "c1_interleave": args.c1_ilv or 16,
             "streams": args.streams,
-            "kernel_table": "warmup steps, every launch timed; roofline kernel timed in the timed steps",
+            "kernel_table": ("warmup steps, every launch timed; the three longest then timed alone over two "
+                             "steps each (roofline_candidates); the roofline kernel timed in the timed steps"),
             "fft_precision": ["fp32", "bf16x3", "bf16", "fp16"][args.precision],
             "parallelism": "1 GPU",
         },
@@ -447,6 +471,7 @@ def main():
             "frac": round(achieved * 1e9 / HBM_PEAK, 4),
             "traffic": measured_traffic(dname, du / dl, args.chunk),
         },
+        "roofline_candidates": roof_cands,
         "pipeline": {
             "block_pols_per_s": round(bp_per_s, 1),
             "b_alg_frac": round(B_ALG_PER_BP * bp_per_s / HBM_PEAK, 4) if not args.rate else None,

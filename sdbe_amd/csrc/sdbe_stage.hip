@@ -991,9 +991,12 @@ struct OutDesc {
 // z[tx + 64 ty] = sum_c Bc[tx][c] e^{2 pi i c tx/2^20} e^{2 pi i c ty/16384} is the packed output:
 // samples (2t, 2t+1) = (Re, Im) z[t], scaled by 2^-36 to numpy units.  Codes
 // per vdif_inout_gpu_thread.cu:577 (v = x - off; ((|v| >= th) | 2) ^ 3 (v < 0));
-// the 4-bit pair code (re | im << 2) of t goes to stage[bp][tx][ty].
+// the 4-bit pair code (re | im << 2) of t goes to the low nibble of
+// stage[bp][tx / 2][ty] for even tx, the high nibble for odd tx.
 // The workgroups are persistent (one per CU: the FFT's LDS fills a CU) and take
-// rows blockIdx.x, blockIdx.x + gridDim.x, ...  A row arrives in stage C1's
+// the rows of pairs blockIdx.x, blockIdx.x + gridDim.x, ... in turn (pair i =
+// rows 2i, 2i + 1, so that a byte's two nibbles are stored once, by the odd
+// row's pass).  A row arrives in stage C1's
 // tile-major slot order (planar: real plane then imaginary plane) and is
 // scattered into LDS in column order (slot_col: column of each slot, -1 for
 // none); the LDS word of each of a thread's slots is computed once.  The next
@@ -1053,7 +1056,11 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             pos2[i >> 1] = p;
     }
     float pr[C2_NSLOT + 1], pi[C2_NSLOT + 1];
-    int row = blockIdx.x;
+    // rows go in pairs (2 i, 2 i + 1) of one block-pol, pair blockIdx.x, + gridDim.x, ...:
+    // the even row's codes wait in hold (byte q & 3 of hold[q >> 2] for element
+    // te + 1024 q) and the odd row stores both as one byte
+    int row = 2 * (int)blockIdx.x;
+    uint32_t hold[4] = {0u, 0u, 0u, 0u};
     if constexpr (P == 0) c2_load(Bc, row, pr, pi);
     for (;;) {
         const int tx = row & (KHI - 1), bpl = row / KHI;
@@ -1065,7 +1072,7 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             lds_put(lds, (i & 1) ? (int)(pos2[i >> 1] >> 16) : (int)(pos2[i >> 1] & 0xffffu),
                     make_float2(pr[i], pi[i]));
         __syncthreads();
-        const int next = row + gridDim.x;
+        const int next = (row & 1) ? row - 1 + 2 * (int)gridDim.x : row + 1;
         float2 v[16];
         if constexpr (P == 0) {
             const int tr = row_tid();
@@ -1115,19 +1122,31 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             }
         } else {
             const float2 to = thr[bpl];
-            // the row's 16384 bytes as a buffer: element te + 1024 q at offset te
-            // (VGPR) + 1024 q (scalar), with no 64-bit address arithmetic per store
-            const __amdgpu_buffer_rsrc_t rd =
-                __builtin_amdgcn_make_buffer_rsrc((void *)(stage + (size_t)row * NCH), 0, NCH, BUF_RSRC_WORD3);
+            uint32_t code[16];
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const float a = v[q].x * scale - to.y;
                 const float b = v[q].y * scale - to.y;
                 // code = (|v| >= th ? 3 : 2) ^ (v < 0 ? 3 : 0): 3 if v >= th, 2 if 0 <= v < th,
                 // 1 if -th < v < 0, 0 if v <= -th; b's code in bits 2-3 (disjoint: XOR = OR)
-                const uint32_t pair = ((fabsf(a) >= to.x) ? 3u : 2u) ^ (a < 0.f ? 3u : 0u) ^
-                                      ((fabsf(b) >= to.x) ? 12u : 8u) ^ (b < 0.f ? 12u : 0u);
-                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)pair, rd, te, 1024 * q, 0);
+                code[q] = ((fabsf(a) >= to.x) ? 3u : 2u) ^ (a < 0.f ? 3u : 0u) ^
+                          ((fabsf(b) >= to.x) ? 12u : 8u) ^ (b < 0.f ? 12u : 0u);
+            }
+            if ((tx & 1) == 0) {   // workgroup-uniform
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    hold[j] = code[4 * j] | (code[4 * j + 1] << 8) | (code[4 * j + 2] << 16) | (code[4 * j + 3] << 24);
+            } else {
+                // stage row tx / 2 of the block-pol, 16384 bytes, as a buffer: element
+                // te + 1024 q at offset te (VGPR) + 1024 q (scalar), with no 64-bit
+                // address arithmetic per store; its low nibble from row tx - 1
+                const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(stage + ((size_t)bpl * (KHI / 2) + (tx >> 1)) * NCH), 0, NCH, BUF_RSRC_WORD3);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const uint32_t m = ((hold[q >> 2] >> (8 * (q & 3))) & 0xFFu) | (code[q] << 4);
+                    __builtin_amdgcn_raw_buffer_store_b8((uint8_t)m, rd, te, 1024 * q, 0);
+                }
             }
         }
         if (next >= n_rows) break;
@@ -1135,16 +1154,16 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
     }
 }
 
-// Pack: the stage holds, per block-pol, 64 rows r of 16384 bytes S[r][ty];
-// byte S[r][ty] holds the 2-bit codes of samples 2p, 2p + 1 (p = r + 64 ty) in
-// its low nibble.  Stream byte q of a block holds samples 4q..4q+3 (sample j at
-// bits 2j, the uint32 packing of vdif_inout_gpu_thread.cu:578-580):
-// byte (i + 32 ty) = S[2i][ty] | S[2i+1][ty] << 4.
-// Thread = (block-pol, columns ty = 4c .. 4c+3): 64 row words in (a wave reads
-// 256 contiguous bytes of a row per load), M_i = S[2i] | S[2i+1] << 4 merges
-// the nibbles bytewise (codes are < 16), and byte j of M_i is stream byte
-// i + 32 (4c + j): the thread's 128 contiguous stream bytes are a 4 x 4 byte
-// transpose of M per output word.  Block bytes go to 64 frames of 8192 bytes;
+// Pack: stage C2's rows r hold the 2-bit codes of samples 2p, 2p + 1
+// (p = r + 64 ty) for ty < 16384; it stores rows 2i and 2i + 1 together as the
+// 32 stage rows M_i of 16384 bytes per block-pol, M_i[ty] = S[2i][ty] |
+// S[2i+1][ty] << 4.  Stream byte q of a block holds samples 4q..4q+3 (sample j
+// at bits 2j, the uint32 packing of vdif_inout_gpu_thread.cu:578-580):
+// byte (i + 32 ty) = M_i[ty].
+// Thread = (block-pol, columns ty = 4c .. 4c+3): 32 row words in (a wave reads
+// 256 contiguous bytes of a row per load), and byte j of word M_i is stream
+// byte i + 32 (4c + j): the thread's 128 contiguous stream bytes are a 4 x 4
+// byte transpose of M per output word.  Block bytes go to 64 frames of 8192 bytes;
 // re-framed output (shift, lead) takes them word by word.
 __device__ __forceinline__ uint32_t pack_word(const uint32_t (&M)[32], int w) {
     const int j = w >> 3, i0 = (w & 7) * 4;
@@ -1155,10 +1174,10 @@ __device__ __forceinline__ uint32_t pack_word(const uint32_t (&M)[32], int w) {
 __global__ void __launch_bounds__(256)
 k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb, OutDesc od) {
     const int bpl = blockIdx.x >> 4, c = ((blockIdx.x & 15) << 8) + threadIdx.x;   // 4096 column quads
-    const GAS uint32_t *src = gptr(reinterpret_cast<const uint32_t *>(stage + (size_t)bpl * KHI * NCH)) + c;
+    const GAS uint32_t *src = gptr(reinterpret_cast<const uint32_t *>(stage + (size_t)bpl * (KHI / 2) * NCH)) + c;
     uint32_t M[32];
 #pragma unroll
-    for (int i = 0; i < 32; ++i) M[i] = src[(size_t)(2 * i) * (NCH / 4)] | (src[(size_t)(2 * i + 1) * (NCH / 4)] << 4);
+    for (int i = 0; i < 32; ++i) M[i] = src[(size_t)i * (NCH / 4)];
     GAS uint8_t *ob = gptr(bpd[bpl].out) + od.hdr_bytes;
     const int64_t qb0 = 128 * (int64_t)c;                 // stream byte of the run, relative to the block
     if (od.shift_bytes == 0 && od.lim_bytes == INT64_MAX && (((uintptr_t)ob) & 15) == 0) {
@@ -1566,8 +1585,8 @@ int build_slot_table(sdbe_handle_t *h) {
 // Run stages A..C2 over block-pols [0, n_bp) in chunks.
 template <typename K>
 void launch_c2_kernel(sdbe_handle_t *h, int nb, hipStream_t s, const BpDesc *bpd, K kern) {
-    const int rows = KHI * nb;
-    const int grid = rows < h->n_cu ? rows : h->n_cu;
+    const int rows = KHI * nb, pairs = rows / 2;
+    const int grid = pairs < h->n_cu ? pairs : h->n_cu;
     LAUNCH(h, KID_C2, nb, s, kern, dim3(grid), dim3(1024), (const float *)h->Bc, (const float2 *)h->thr, bpd,
            h->stage, (const int *)h->slot_col, h->tb, rows);
 }
@@ -1972,7 +1991,7 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->slot_col, C2_SLOT_TAB * sizeof(int));
     ALLOC(h->part, C * C1_TILES * 2 * sizeof(double));
     ALLOC(h->thr, C * sizeof(float2));
-    ALLOC(h->stage, C * KHI * NCH);
+    ALLOC(h->stage, C * (KHI / 2) * NCH);
     ALLOC(h->ms, C * sizeof(double2));
 #undef ALLOC
     if (hipHostMalloc((void **)&h->h_cnt, CNT_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
@@ -2437,7 +2456,7 @@ constexpr size_t NA_V = 0;
 constexpr size_t NA_ZF = NA_V + na_al((size_t)NA_CHUNK * NBLK * NCH * sizeof(float2));
 constexpr size_t NA_BC = NA_ZF + na_al((size_t)NA_CHUNK * MP * sizeof(float2));
 constexpr size_t NA_STAGE = NA_BC + na_al((size_t)NA_CHUNK * KHI * BC_ROW * sizeof(float2));
-constexpr size_t NA_PART = NA_STAGE + na_al((size_t)NA_CHUNK * KHI * NCH);
+constexpr size_t NA_PART = NA_STAGE + na_al((size_t)NA_CHUNK * (KHI / 2) * NCH);
 constexpr size_t NA_THR = NA_PART + na_al((size_t)NA_CHUNK * C1_TILES * 2 * sizeof(double));
 constexpr size_t NA_MS = NA_THR + na_al((size_t)NA_CHUNK * sizeof(float2));
 constexpr size_t NA_SLOT = NA_MS + na_al((size_t)NA_CHUNK * sizeof(double2));

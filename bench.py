@@ -41,10 +41,11 @@ ALG_BYTES = {
     "k_stage_b": 2 * 38 * 16384 * 8,                     # V rows 1..38 in, packed R2C rows out
     "k_stage_c1": 39 * 16384 * 8 + (1 << 20) * 8,        # packed R2C in, first C2R pass out
     "k_stage_c2": (1 << 20) * 8 + (1 << 19),             # second pass in, 2-bit codes out (4 per byte)
-    "k_pack": (1 << 19) + 64 * 8192,                     # codes in, payloads out
+    "k_pack": (1 << 19) + 64 * 8192 + 64 * 32,           # codes in, payloads and frame headers out
     "k_headers": 64 * 32,
     "k_stats": 513 * 16,
     "k_fill_i32": 79872 * 4,
+    "k_clear_map": 79872 * 4,                            # packet map cleared (and 8 counters)
 }
 B_ALG_PER_BP = 48304512          # stage-boundary algorithmic bytes per block-pol
 COMPULSORY_PER_BP = 329472 + 526336
@@ -316,7 +317,7 @@ def main():
     ppf, pay = (256, 8192) if args.geometry == "8x1024" else (2048, 1024)
     if args.geometry == "8x1024":
         npk = ppf * frames
-        alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4,
+        alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4, "k_clear_map": npk * 4,
                           "k_reorder_8x1024": npk * pay + 2 * 128 * frames * 32768,
                           "k_group_bitmap_8x1024": npk * 4 + frames * 16 * 2})
     if args.rate:
@@ -325,7 +326,7 @@ def main():
         d, e = st.snaps, st.frames
         p = 256 * e
         npk = ppf * d           # packets of a group of D B-frames
-        alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4,
+        alg_bytes.update({"k_packet_map": npk * 32, "k_fill_i32": npk * 4, "k_clear_map": npk * 4,
                           "k_reorder": npk * pay + 2 * 128 * d * 32768,
                           "k_reorder_8x1024": npk * pay + 2 * 128 * d * 32768,
                           "k_trim_c2r": d * 16384 * 2 + d * 32768 * 4,

@@ -24,7 +24,7 @@
 //   k_stats      per block-pol mean / ddof=1 sigma in FP64, threshold policy
 //   k_stage_c2   second (16384-point) pass, scaling, 2-bit quantisation and
 //                packing straight into the VDIF payloads
-//   k_headers    VDIF output headers (FRAMES mode)
+//   k_headers    VDIF output headers (FRAMES mode; at 64/39 k_pack stamps them)
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -86,6 +86,13 @@ __device__ __forceinline__ GAS T *gptr(T *p) {
 __global__ void __launch_bounds__(256) k_fill_i32(int *p, int n, int val) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = val;
+}
+
+// A group's packet map to MAP_NONE and its counters to zero, in one launch.
+__global__ void __launch_bounds__(256) k_clear_map(int *__restrict__ map, int n, int *__restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) map[i] = MAP_NONE;
+    if (i < CNT_N) cnt[i] = 0;
 }
 
 // One thread per packet: header words 0, 4, 5 give invalid, B-count, cid, fid
@@ -891,8 +898,11 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
 // 32768 * 2^21 = 2^36 times numpy's; Y here carries a factor 32768.
 //   sum x^2 = W / (32768^2 * L),  mean = Re Y[0] / (32768 * L),
 //   sigma = sqrt((sum x^2 - L mean^2) / (L - 1))
+// PER_BLOCK and FIXED thresholds depend on the block-pol alone and are written
+// here; the REFERENCE walk (k_thresh) runs only for chunks that need it.
 __global__ void __launch_bounds__(256)
-k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, double *__restrict__ stats_out) {
+k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, double *__restrict__ stats_out,
+        const BpDesc *__restrict__ bpd, float2 *__restrict__ thr) {
     __shared__ double red[2][256];
     const int t = threadIdx.x, b = blockIdx.x;
     const double L = 2097152.0, FA = 32768.0;
@@ -918,6 +928,10 @@ k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, doub
         if (var < 0.0) var = 0.0;
         const double sig = sqrt(var);
         ms[b] = make_double2(mean, sig);
+        const BpDesc &d = bpd[b];
+        if (d.policy != SDBE_THRESH_REFERENCE)
+            thr[b] = d.policy == SDBE_THRESH_PER_BLOCK ? make_float2((float)sig, (float)mean)
+                                                       : make_float2(d.fixed_th, d.fixed_off);
         if (stats_out) {
             stats_out[(size_t)(bp0 + b) * 2] = mean;
             stats_out[(size_t)(bp0 + b) * 2 + 1] = sig;
@@ -976,7 +990,7 @@ __global__ void __launch_bounds__(256) k_thresh(const double2 *__restrict__ ms, 
 struct OutDesc {
     int32_t frame_bytes;     // 8192 (PAYLOAD) or 8224 (FRAMES)
     int32_t hdr_bytes;       // 0 or 32
-    int32_t bp_headers;      // stamp 64 frames per block-pol (k_headers)
+    int32_t bp_headers;      // stamp the block-pols' frame headers (k_pack at 64/39, k_headers band-trimmed)
     int32_t reserved;
     // re-framed group output (output-thread time alignment): block blk's
     // stream byte q goes to group stream byte blk * 524288 + q + shift_bytes;
@@ -1165,6 +1179,23 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
 // byte i + 32 (4c + j): the thread's 128 contiguous stream bytes are a 4 x 4
 // byte transpose of M per output word.  Block bytes go to 64 frames of 8192 bytes;
 // re-framed output (shift, lead) takes them word by word.
+// VDIF output headers (vdif_out_databuf.h:19-52) stamped as the output thread
+// does (vdif_out_net_thread.c:294-317, defaults vdif_out_databuf.c:195-227):
+// frame n (of fpb per block-pol) of a block-pol whose first frame is (secs, df, psn) has
+// df = (df + n) mod 125000, secs + carry, psn + n; threadID 0, bps = 1,
+// df_len = 8224 / 8, edv5 = 2 (words 3 and 4 come prebuilt per block-pol).
+// k_headers stamps them in a launch of its own; at 64/39 k_pack stamps its
+// block-pols' headers while it packs their payloads.
+__device__ __forceinline__ void stamp_header(const BpDesc &d, int n, int frame_bytes) {
+    const uint64_t dft = (uint64_t)d.df + (uint64_t)n;
+    const uint32_t df = (uint32_t)(dft % 125000u);
+    const uint32_t secs = d.secs + (uint32_t)(dft / 125000u);
+    const uint64_t psn = d.psn + (uint64_t)n;
+    GAS u32x4 *dst = reinterpret_cast<GAS u32x4 *>(gptr(d.out) + (int64_t)n * frame_bytes);
+    dst[0] = u32x4{secs & 0x3FFFFFFFu, (df & 0xFFFFFFu) | d.w1hi, (uint32_t)((OUT_FRAME_PAYLOAD + 32) / 8), d.w3};
+    dst[1] = u32x4{d.w4, 0u, (uint32_t)(psn & 0xFFFFFFFFu), (uint32_t)(psn >> 32)};
+}
+
 __device__ __forceinline__ uint32_t pack_word(const uint32_t (&M)[32], int w) {
     const int j = w >> 3, i0 = (w & 7) * 4;
     return ((M[i0] >> (8 * j)) & 0xFFu) | (((M[i0 + 1] >> (8 * j)) & 0xFFu) << 8) |
@@ -1174,6 +1205,9 @@ __device__ __forceinline__ uint32_t pack_word(const uint32_t (&M)[32], int w) {
 __global__ void __launch_bounds__(256)
 k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb, OutDesc od) {
     const int bpl = blockIdx.x >> 4, c = ((blockIdx.x & 15) << 8) + threadIdx.x;   // 4096 column quads
+    // the block-pol's 64 frame headers (FRAMES mode), by its first workgroup
+    if (od.hdr_bytes && od.bp_headers && (blockIdx.x & 15) == 0 && threadIdx.x < OUT_FRAMES_BLOCK)
+        stamp_header(bpd[bpl], (int)threadIdx.x, od.frame_bytes);
     const GAS uint32_t *src = gptr(reinterpret_cast<const uint32_t *>(stage + (size_t)bpl * (KHI / 2) * NCH)) + c;
     uint32_t M[32];
 #pragma unroll
@@ -1220,23 +1254,11 @@ k_pack(const uint8_t *__restrict__ stage, const BpDesc *__restrict__ bpd, int nb
     }
 }
 
-// VDIF output headers (vdif_out_databuf.h:19-52) stamped as the output thread
-// does (vdif_out_net_thread.c:294-317, defaults vdif_out_databuf.c:195-227):
-// frame n (of fpb per block-pol) of a block-pol whose first frame is (secs, df, psn) has
-// df = (df + n) mod 125000, secs + carry, psn + n; threadID 0, bps = 1,
-// df_len = 8224 / 8, edv5 = 2 (words 3 and 4 come prebuilt per block-pol).
 __global__ void __launch_bounds__(256) k_headers(const BpDesc *__restrict__ bpd, int n_bp, OutDesc od, int fpb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_bp * fpb) return;
     const int b = i / fpb, n = i - b * fpb;
-    const BpDesc &d = bpd[b];
-    const uint64_t dft = (uint64_t)d.df + (uint64_t)n;
-    const uint32_t df = (uint32_t)(dft % 125000u);
-    const uint32_t secs = d.secs + (uint32_t)(dft / 125000u);
-    const uint64_t psn = d.psn + (uint64_t)n;
-    GAS u32x4 *dst = reinterpret_cast<GAS u32x4 *>(gptr(d.out) + (int64_t)n * od.frame_bytes);
-    dst[0] = u32x4{secs & 0x3FFFFFFFu, (df & 0xFFFFFFu) | d.w1hi, (uint32_t)((OUT_FRAME_PAYLOAD + 32) / 8), d.w3};
-    dst[1] = u32x4{d.w4, 0u, (uint32_t)(psn & 0xFFFFFFFFu), (uint32_t)(psn >> 32)};
+    stamp_header(bpd[b], n, od.frame_bytes);
 }
 
 // Headers of n frames per polarisation of a re-framed group: frame n of pol p
@@ -1319,13 +1341,13 @@ __global__ void __launch_bounds__(64) k_spectra_desc(SpecDescArgs a, int n_bp, B
 enum KernelId {
     KID_FILL, KID_MAP, KID_REORDER, KID_A, KID_B, KID_C1, KID_STATS, KID_THRESH, KID_C2,
     KID_PACK, KID_HEADERS, KID_TRIM_C2R, KID_TRIM_SEG, KID_TRIM_PACK, KID_A_PK, KID_BITMAP,
-    KID_REORDER8, KID_A_PK8, KID_BITMAP8, KID_TRIM_C2R_PK, KID_N
+    KID_REORDER8, KID_A_PK8, KID_BITMAP8, KID_TRIM_C2R_PK, KID_CLEAR, KID_N
 };
 static const char *const kKernelNames[KID_N] = {
     "k_fill_i32", "k_packet_map", "k_reorder", "k_stage_a", "k_stage_b",
     "k_stage_c1", "k_stats", "k_thresh", "k_stage_c2", "k_pack", "k_headers", "k_trim_c2r", "k_trim_seg",
     "k_trim_pack", "k_stage_a_pk", "k_group_bitmap", "k_reorder_8x1024", "k_stage_a_pk8",
-    "k_group_bitmap_8x1024", "k_trim_c2r_pk"};
+    "k_group_bitmap_8x1024", "k_trim_c2r_pk", "k_clear_map"};
 
 struct ProfRec {
     int kid;
@@ -1650,7 +1672,7 @@ int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od,
 // per-snapshot C2R) reads them directly (the block-pols' blk / pol then locate
 // their snapshots through the packet map), else null.
 int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od, bool f32,
-                  double *d_stats, hipStream_t s, const uint32_t *pk = nullptr) {
+                  double *d_stats, hipStream_t s, const uint32_t *pk = nullptr, bool ref_walk = true) {
     if (h->trim.frac == 6) return run_trim<39, 32>(h, d_bpd, n_bp, od, f32, d_stats, s, pk);
     if (h->trim.frac == 8) return run_trim<13, 8>(h, d_bpd, n_bp, od, f32, d_stats, s, pk);
     if (h->trim.frac == 10) return run_trim<65, 64>(h, d_bpd, n_bp, od, f32, d_stats, s, pk);
@@ -1693,19 +1715,18 @@ int run_resampler(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc
         const int c1_wgs = (C1_TILES * nb + per_grp - 1) / per_grp * h->c1_ilv;
         LAUNCH(h, KID_C1, nb, s, k_stage_c1, dim3(c1_wgs), dim3(256), (const float2 *)h->Zf, (float *)h->Bc, h->part, h->tb,
                nb, h->c1_tpw, h->c1_ilv);
-        LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats);
-        LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(256), (const double2 *)h->ms, nb, bpd, h->latch, h->thr);
+        LAUNCH(h, KID_STATS, nb, s, k_stats, dim3(nb), dim3(256), (const double *)h->part, bp0, h->ms, d_stats, bpd,
+               h->thr);
+        if (ref_walk)   // some block-pol of the call has the REFERENCE policy
+            LAUNCH(h, KID_THRESH, nb, s, k_thresh, dim3(1), dim3(256), (const double2 *)h->ms, nb, bpd, h->latch,
+                   h->thr);
         if (f32) {
             launch_c2<true>(h, nb, s, bpd);
         } else {
             launch_c2<false>(h, nb, s, bpd);
+            // with the block-pols' headers (FRAMES mode)
             LAUNCH(h, KID_PACK, nb, s, k_pack, dim3(16 * nb), dim3(256), (const uint8_t *)h->stage, bpd, nb, od);
         }
-        HIP_TRY(h, hipGetLastError());
-    }
-    if (!f32 && od.hdr_bytes && od.bp_headers) {
-        LAUNCH(h, KID_HEADERS, n_bp, s, k_headers, dim3((n_bp * OUT_FRAMES_BLOCK + 255) / 256), dim3(256), d_bpd,
-               n_bp, od, OUT_FRAMES_BLOCK);
         HIP_TRY(h, hipGetLastError());
     }
     return SDBE_OK;
@@ -1777,8 +1798,7 @@ sdbe_stream_state_t state_plus(const sdbe_stream_state_t &st, int64_t n_frames) 
 // the int8 rows (k_reorder) or, for the packet-reading stage A, only the fid
 // bitmap and the unique count.
 int unpack_map(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uint64_t b_zero, hipStream_t s) {
-    LAUNCH(h, KID_FILL, 1, s, k_fill_i32, dim3((h->map_n + 255) / 256), dim3(256), h->map, h->map_n, MAP_NONE);
-    LAUNCH(h, KID_FILL, 0, s, k_fill_i32, dim3(1), dim3(256), h->cnt, CNT_N, 0);
+    LAUNCH(h, KID_CLEAR, 1, s, k_clear_map, dim3((h->map_n + 255) / 256), dim3(256), h->map, h->map_n, h->cnt);
     if (n_pkts > 0)
         LAUNCH(h, KID_MAP, 1, s, k_packet_map, dim3((n_pkts + 255) / 256), dim3(256), (const uint32_t *)d_pkts,
                n_pkts, (uint32_t)(b_zero & 0xFFFFFFFFull), h->nfr, h->n_cid, h->n_fid, h->pkt_words, h->map,
@@ -1824,8 +1844,8 @@ void advance_state(sdbe_stream_state_t &st, int64_t n_frames) {
 }
 
 int finish_call(sdbe_handle_t *h, hipStream_t s) {
-    HIP_TRY(h, hipMemcpyAsync(h->h_cnt, h->cnt, CNT_N * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(h, hipMemcpyAsync(h->h_latch, h->latch, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    // the counters and latch slot 0 lie back to back on both sides: one copy
+    HIP_TRY(h, hipMemcpyAsync(h->h_cnt, h->cnt, CNT_N * sizeof(int) + 2 * sizeof(float), hipMemcpyDeviceToHost, s));
     h->last_stream = s;
     return SDBE_OK;
 }
@@ -1871,8 +1891,7 @@ void sdbe_destroy(sdbe_handle_t *h) {
     (void)hipFree(h->map);
     (void)hipFree(h->beng);
     (void)hipFree(h->bitmap);
-    (void)hipFree(h->cnt);
-    (void)hipFree(h->latch);
+    (void)hipFree(h->cnt);   // the latch slots too
     (void)hipFree(h->V);
     (void)hipFree(h->Zf);
     (void)hipFree(h->Bc);
@@ -1882,8 +1901,7 @@ void sdbe_destroy(sdbe_handle_t *h) {
     (void)hipFree(h->slot_col);
     (void)hipFree(h->trim_tab);
     (void)hipFree(h->ms);
-    (void)hipHostFree(h->h_cnt);
-    (void)hipHostFree(h->h_latch);
+    (void)hipHostFree(h->h_cnt);   // and h_latch
     for (int i = 0; i < sdbe_handle::RING; ++i) {
         (void)hipFree(h->d_bpd[i]);
         (void)hipHostFree(h->h_bpd[i]);
@@ -1976,8 +1994,9 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->map, (size_t)h->map_n * sizeof(int));
     ALLOC(h->beng, 2 * (size_t)h->nfr * 128 * NCH * 2);
     ALLOC(h->bitmap, (size_t)h->nfr * 256);
-    ALLOC(h->cnt, CNT_N * sizeof(int));
-    ALLOC(h->latch, 2 * SDBE_LATCH_SLOTS * sizeof(float));
+    // the latch slots follow the counters in one allocation (read back together)
+    ALLOC(h->cnt, CNT_N * sizeof(int) + 2 * SDBE_LATCH_SLOTS * sizeof(float));
+    h->latch = reinterpret_cast<float *>(h->cnt + CNT_N);
     for (int i = 0; i < sdbe_handle::RING; ++i) ALLOC(h->d_bpd[i], (size_t)h->cap_bp * sizeof(BpDesc));
     ALLOC(h->g_d, (size_t)h->cap_bp * sizeof(BpDesc));
     ALLOC(h->lead, 2 * OUT_FRAME_PAYLOAD);
@@ -1994,11 +2013,12 @@ int sdbe_create(const sdbe_config_t *cfg_in, int device, sdbe_handle_t **out) {
     ALLOC(h->stage, C * (KHI / 2) * NCH);
     ALLOC(h->ms, C * sizeof(double2));
 #undef ALLOC
-    if (hipHostMalloc((void **)&h->h_cnt, CNT_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_latch, 2 * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&h->h_cnt, CNT_N * sizeof(int) + 2 * sizeof(float), hipHostMallocDefault) !=
+        hipSuccess) {
         rc = SDBE_ERR_NOMEM;
         goto fail;
     }
+    h->h_latch = reinterpret_cast<float *>(h->h_cnt + CNT_N);
     for (int i = 0; i < sdbe_handle::RING; ++i) {
         if (hipHostMalloc((void **)&h->h_bpd[i], (size_t)h->cap_bp * sizeof(BpDesc), hipHostMallocDefault) !=
                 hipSuccess ||
@@ -2153,8 +2173,8 @@ int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts, uin
         od.lead = h->lead;
         od.bp_headers = 0;
     }
-    rc = run_resampler(h, h->d_bpd[slot], n_bp, od, false, nullptr, s,
-                       direct ? (const uint32_t *)d_pkts : nullptr);
+    rc = run_resampler(h, h->d_bpd[slot], n_bp, od, false, nullptr, s, direct ? (const uint32_t *)d_pkts : nullptr,
+                       c.thresh_policy == SDBE_THRESH_REFERENCE);
     if (rc) return rc;
     if ((rc = ring_release(h, slot, s))) return rc;
     if (!od.bp_headers && od.hdr_bytes) {
@@ -2195,7 +2215,8 @@ int spectra_call(sdbe_handle_t *h, const int8_t *d_beng, int32_t n_bp, uint8_t *
                c.fixed_offset, c.station_id, c.bdc_sideband, c.rx_sideband);
     if ((rc = ring_upload(h, slot, n_bp, s))) return rc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h->cnt, CNT_N, 0);
-    rc = run_resampler(h, h->d_bpd[slot], n_bp, out_desc(h), f32, d_stats, s);
+    rc = run_resampler(h, h->d_bpd[slot], n_bp, out_desc(h), f32, d_stats, s, nullptr,
+                       c.thresh_policy == SDBE_THRESH_REFERENCE);
     if (rc) return rc;
     return ring_release(h, slot, s);
 }
@@ -2330,7 +2351,11 @@ int sdbe_process_blocks(sdbe_handle_t *h, const sdbe_stream_desc_t *streams, int
             fill_batch(h, streams, n_streams, h->h_bpd[slot]);
             if ((rc = ring_upload(h, slot, n_bp, s))) return rc;
             hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, h->cnt, CNT_N, 0);
-            if ((rc = run_resampler(h, h->d_bpd[slot], n_bp, out_desc(h), false, nullptr, s))) return rc;
+            bool any_ref = false;
+            for (int i = 0; i < n_streams; ++i)
+                any_ref |= streams[i].n_blocks > 0 && streams[i].thresh_policy == SDBE_THRESH_REFERENCE;
+            if ((rc = run_resampler(h, h->d_bpd[slot], n_bp, out_desc(h), false, nullptr, s, nullptr, any_ref)))
+                return rc;
             if ((rc = ring_release(h, slot, s))) return rc;
         }
     }
@@ -2576,7 +2601,7 @@ extern "C" int sdbe_process_spectra_noalloc(const sdbe_config_t *cfg, sdbe_strea
     hipLaunchKernelGGL(k_spectra_desc, dim3((n_bp + 63) / 64), dim3(64), 0, s, a, n_bp, bpd);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return SDBE_ERR_HIP;
-    rc = run_resampler(&h, bpd, n_bp, od, false, nullptr, s);
+    rc = run_resampler(&h, bpd, n_bp, od, false, nullptr, s, nullptr, cfg->thresh_policy == SDBE_THRESH_REFERENCE);
     if (rc) return rc;
     advance_state(*state, (int64_t)n_bp * OUT_FRAMES_BLOCK);
     return SDBE_OK;

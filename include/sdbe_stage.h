@@ -66,7 +66,7 @@ extern "C" {
 #define SDBE_OUT_HEADER_BYTES    32
 #define SDBE_OUT_FRAMES_PER_GROUP (SDBE_BLOCKS_PER_GROUP * SDBE_OUT_FRAMES_PER_BLOCK)
 #define SDBE_MAX_CHUNK           1024   /* block-pols per pipeline chunk, at most
-                                           (about 21 GB of working buffers at 64/39) */
+                                           (about 20.5 GB of working buffers at 64/39) */
 
 /* ---- threshold policy --------------------------------------------------- */
 /* REFERENCE: mean and ddof=1 sigma of the first block of polarisation 0 are

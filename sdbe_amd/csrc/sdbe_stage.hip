@@ -1645,10 +1645,12 @@ int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od,
             LAUNCH(h, KID_TRIM_C2R, nb, s, kc2r, dim3(D * nb), dim3(1024), bpd, h->V, h->tb);
         if (f32)
             LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_f, dim3(trim::NSEG * nb), dim3(trim::seg_threads(D)),
-                   (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip);
+                   (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip,
+                   (const float2 *)h->tb.fft);
         else
             LAUNCH(h, KID_TRIM_SEG, nb, s, kseg_c, dim3(trim::NSEG * nb), dim3(trim::seg_threads(D)),
-                   (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip);
+                   (const float *)h->V, bpd, (float *)h->Zf, h->part, h->trim_tabs, h->trim.skip,
+                   (const float2 *)h->tb.fft);
         LAUNCH(h, KID_STATS, nb, s, k_trim_stats, dim3(nb), dim3(trim::NSEG), (const double *)h->part, P, bp0, h->ms,
                d_stats);
         if (!f32) {

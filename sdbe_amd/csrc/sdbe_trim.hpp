@@ -366,13 +366,23 @@ k_trim_c2r_pk(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb,
 template <int D, int E, bool F32>
 __global__ void __launch_bounds__(trim::seg_threads(D))
 k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *__restrict__ stage,
-           double *__restrict__ part, trim::Tabs tt, int skip) {
+           double *__restrict__ part, trim::Tabs tt, int skip, const float2 *__restrict__ fft_tw) {
     constexpr int R = 256 * D, P = 256 * E, NR = R / 2, NP = P / 2;
     constexpr int NBUF = NR > NP ? NR : NP;
     constexpr int NT = trim::seg_threads(D);
+    // 11/11: the C2R is the 16384-point transform of stages B and C2 on 1024
+    // threads (registers and a 128 KB work array inside buf, its twiddles in LDS)
+    constexpr bool F16K = NP == NCH && NT == 1024;
+    static_assert(!F16K || NBUF * 2 >= LDS_FFT_N, "work array");
     // one in-place buffer: 39 KB at 6/11 (four workgroups per CU), 146 KB at 11/11
     __shared__ float2 buf[NBUF];
     __shared__ double red[2][NT / 64];
+    const FftTw *ftwp = nullptr;
+    if constexpr (F16K) {
+        __shared__ FftTw ftw;
+        fft_tw_load(&ftw, fft_tw);
+        ftwp = &ftw;
+    }
     const int bpl = blockIdx.x / trim::NSEG, g = blockIdx.x - bpl * trim::NSEG, t = threadIdx.x;
     const float2 *z = reinterpret_cast<const float2 *>(T + (size_t)bpl * D * 2 * NCH + (size_t)g * R);
     for (int m = t; m < NR; m += NT) buf[trim::swz16(m)] = z[m];
@@ -399,23 +409,37 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
         const float2 sum = cadd(y, cconj(ym)), dif = csub(y, cconj(ym));
         wv[q] = cadd(sum, mul_si<+1>(cmul(tt.wp[j], dif)));
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < JT; ++q) buf[trim::swz16(t + NT * q)] = wv[q];
-    __syncthreads();
-    trim::fft<NP, +1, NT>(buf, tt.zp);
-    const float2 *w = buf;
     // samples (2m, 2m+1) = (Re, Im) w[m] / (32768 P)
     const float sc = 1.0f / (32768.0f * (float)P);
     float2 *dst = reinterpret_cast<float2 *>((F32 ? (float *)bpd[bpl].out : stage + (size_t)bpl * trim::NSEG * P) +
                                              (size_t)g * P);
     double s1 = 0.0, s2 = 0.0;
+    if constexpr (F16K) {
+        // wv[q] is element t + 1024 q, fft16k_t's input layout; it waits at its
+        // first barrier until every read of Z is done
+        fft16k_t<+1>(wv, reinterpret_cast<float *>(buf), ftwp);   // element fft16k_tord(t) + 1024 q in wv[q]
+        float2 *d = dst + fft16k_tord(t);
+#pragma unroll
+        for (int q = 0; q < JT; ++q) {
+            const float2 x = cscale(wv[q], sc);
+            d[1024 * q] = x;
+            s1 += (double)x.x + (double)x.y;
+            s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+        }
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < JT; ++q) buf[trim::swz16(t + NT * q)] = wv[q];
+        __syncthreads();
+        trim::fft<NP, +1, NT>(buf, tt.zp);
+        const float2 *w = buf;
 #pragma unroll 4
-    for (int m = t; m < NP; m += NT) {
-        const float2 x = cscale(w[trim::swz16(m)], sc);
-        dst[m] = x;
-        s1 += (double)x.x + (double)x.y;
-        s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+        for (int m = t; m < NP; m += NT) {
+            const float2 x = cscale(w[trim::swz16(m)], sc);
+            dst[m] = x;
+            s1 += (double)x.x + (double)x.y;
+            s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

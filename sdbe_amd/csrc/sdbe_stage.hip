@@ -2,18 +2,20 @@
 // of include/sdbe_stage.h.
 //
 // Pipeline per group (39 B-frames, both polarisations, 256 block-pols):
-//   k_fill_i32 + k_packet_map   packet index map [frame][cid][fid], first packet
+//   k_clear_map + k_packet_map  packet index map [frame][cid][fid], first packet
 //                               wins; invalid / out-of-window packets counted
 //   k_reorder                   LDS transpose of 16 packets per (frame, cid pair)
 //                               into the reference int8 [snapshot][channel][im,re]
 //                               rows, missing packets -> 0 + 0j, fid bitmap
-//                               (int8 unpack entry point, band-trimmed rates)
-//   k_group_bitmap              fid bitmap and unique count only (64/39 groups,
-//                               whose stage A reads the packets)
+//                               (int8 unpack entry point, spectra calls)
+//   k_group_bitmap              fid bitmap and unique count only (groups whose
+//                               first kernel reads the packets)
+//   (k_reorder_8x1024, k_group_bitmap_8x1024: the same for 8 x 1024 packets)
 // then per chunk of block-pols:
 //   k_stage_a    decode + C2R pre-processing of each 32768-point snapshot
 //                transform + 39-point DFT across the block's snapshots
-//                (k_stage_a_pk: the same, decoding the group's packets in place)
+//                (k_stage_a_pk / k_stage_a_pk8: the same, decoding the group's
+//                packets in place)
 //   k_stage_b    per 39-DFT row k1 = 1..38: 16384-point inverse FFT, inter-pass
 //                twiddle, 16384-point forward FFT  => packed 1,277,952-point R2C
 //                (row 0, whose twiddle is 1, is 2^14 times stage A's row 0 and is
@@ -21,10 +23,12 @@
 //   k_stage_c1   R2C post-processing, zero padding, C2R pre-processing of the
 //                2,097,152-point output transform, first (64-point) pass of
 //                its packed 2^20 inverse FFT; FP64 Parseval partial sums
-//   k_stats      per block-pol mean / ddof=1 sigma in FP64, threshold policy
-//   k_stage_c2   second (16384-point) pass, scaling, 2-bit quantisation and
-//                packing straight into the VDIF payloads
-//   k_headers    VDIF output headers (FRAMES mode; at 64/39 k_pack stamps them)
+//   k_stats      per block-pol mean / ddof=1 sigma in FP64, PER_BLOCK and FIXED
+//                thresholds (k_thresh: the REFERENCE latch walk, when needed)
+//   k_stage_c2   second (16384-point) pass, scaling, 2-bit quantisation, rows in
+//                pairs so that each stage byte holds four codes
+//   k_pack       stage bytes to VDIF payloads in stream order, frame headers
+// (band-trimmed rates: sdbe_trim.hpp)
 #include <hip/hip_runtime.h>
 
 #include <math.h>

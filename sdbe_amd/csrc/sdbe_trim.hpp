@@ -29,10 +29,14 @@
 namespace trim {
 
 constexpr int NSEG = 128;             // segments per block (D 32768 / (256 D))
-// Threads of a segment workgroup: 256 while the R/2-point buffer is small
-// (6/11, 8/11: up to 39 KB, four workgroups per CU), 1024 for the 66 KB
-// (10/11) and 146 KB (11/11) buffers, where one or two workgroups fill a CU.
-__host__ __device__ constexpr int seg_threads(int D) { return 128 * D > 5000 ? 1024 : 256; }
+// Threads of a segment workgroup, sized to the butterfly counts of its passes
+// (measured): 128 at 8/11 (13 KB buffer; passes of 64 to 208 butterflies), 256
+// at 6/11 (39 KB, four workgroups per CU), 512 at 10/11 (66 KB, two per CU;
+// radix-16 passes of 512 and 520 butterflies), 1024 at 11/11 (146 KB), whose
+// C2R is the 16384-point transform of stages B and C2.
+__host__ __device__ constexpr int seg_threads(int D) {
+    return D > 100 ? 1024 : (D > 40 ? 512 : (D > 20 ? 256 : 128));
+}
 
 // Host: geometry of a preset.  Returns false for an unsupported preset or a
 // skip that leaves fewer than P/2 + 1 bins.

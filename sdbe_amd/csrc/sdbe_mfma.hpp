@@ -131,6 +131,23 @@ __device__ __forceinline__ int mfma_elem(int g, int j, int t) {
     return 64 * w + 16 * g + (l & 15) + 1024 * (4 * (l >> 4) + j);
 }
 
+// LDS words of the MFMA layouts, from one opaque base per use (lds_opaque), so
+// that the compiler does not keep sixteen addresses of every pass live across
+// the transform.  lds_re is XOR-linear over element bits 0-9 and passes bits
+// 10-13 through doubled, so with lane (w, n, kb):
+//   input element 64 w + 16 g + n + 1024 (4 kb + j)  ->  (rb ^ lds_re(16 g)) + 2048 j,
+//     rb = lds_re(64 w + n) | 8192 kb
+//   pass output, Ns = 1:    base lds_re(16 n + 4 kb) | 2048 w,              ^ lds_re(256 g) ^ lds_re(i)
+//   pass output, Ns = 16:   base lds_re(64 kb + n) | 2048 w,                ^ lds_re(256 g) ^ lds_re(16 i)
+//   pass output, Ns = 256:  base lds_re(64 (w & 3) + n) | 8192 (w >> 2) | 2048 kb,
+//                                                                           ^ lds_re(16 g) ^ lds_re(256 i)
+// (the constants are below 2048, the bases' high parts above).
+__device__ __forceinline__ int mfma_rd_base(int t) {
+    const int w = t >> 6, l = t & 63;
+    return lds_opaque(lds_re(64 * w + (l & 15)) | (8192 * (l >> 4)));
+}
+__device__ __forceinline__ int mfma_rd(int rb, int g, int j) { return (rb ^ lds_re(16 * g)) + 2048 * j; }
+
 // Radix-16 passes on the matrix cores, then the radix-4 VALU pass.
 // v: the input in MFMA layout (v[4 g + j] = element mfma_elem(g, j));
 // on return v[q] = element threadIdx.x + 1024 q of the transform.
@@ -170,23 +187,29 @@ __device__ __forceinline__ void fft16k_mfma(float2 *v, float *lds, const FftTw *
 #pragma unroll
         for (int g = 0; g < 4; ++g) mfma_dft16<S, P>(v + 4 * g, W);
         __syncthreads();   // previous readers of the LDS array are done
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int d = 64 * w + 16 * g + n;
-            const int base = (d / ns) * 16 * ns + (d % ns);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                lds_put(lds, lds_re(base + ns * (4 * kb + i)), v[4 * g + i]);
-            }
-        }
-        __syncthreads();
-        if (pass < 2) {
+        {
+            // output k = 4 kb + i of DFT d = 64 w + 16 g + n goes to element
+            // (d / Ns) 16 Ns + (d % Ns) + Ns k
+            const int wb = lds_opaque(pass == 0   ? (lds_re(16 * n + 4 * kb) | (2048 * w))
+                                      : pass == 1 ? (lds_re(64 * kb + n) | (2048 * w))
+                                                  : (lds_re(64 * (w & 3) + n) | (8192 * (w >> 2)) | (2048 * kb)));
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    v[4 * g + j] = lds_get(lds, lds_re(mfma_elem(g, j, t)));
+                for (int i = 0; i < 4; ++i) {
+                    const int c = pass == 0   ? (lds_re(256 * g) ^ lds_re(i))
+                                  : pass == 1 ? (lds_re(256 * g) ^ lds_re(16 * i))
+                                              : (lds_re(16 * g) ^ lds_re(256 * i));
+                    lds_put(lds, wb ^ c, v[4 * g + i]);
                 }
+        }
+        __syncthreads();
+        if (pass < 2) {
+            const int rb = mfma_rd_base(t);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[4 * g + j] = lds_get(lds, mfma_rd(rb, g, j));
         }
     }
     // radix 4, Ns = 4096 (as in fft16k): element t + 1024 (b + 4 r) -> v[b + 4 r]

@@ -634,11 +634,18 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             MfmaW<+1, P> W;
             W.init(t);
             {
-                const float2 *src = V + ((size_t)bpl * NBLK + k1) * NCH;
+                // element 64 w + 16 g + n + 1024 (4 kb + j) of the row: one VGPR offset,
+                // g in the immediate offset, j in the scalar offset
+                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(V + ((size_t)bpl * NBLK + k1) * NCH), 0, NCH * (int)sizeof(float2), BUF_RSRC_WORD3);
+                const int l = t & 63;
+                const int vo = 8 * (64 * (t >> 6) + (l & 15) + 4096 * (l >> 4));
 #pragma unroll
                 for (int g = 0; g < 4; ++g)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) v[4 * g + j] = src[mfma_elem(g, j, t)];
+                    for (int j = 0; j < 4; ++j)
+                        v[4 * g + j] = __builtin_bit_cast(
+                            float2, __builtin_amdgcn_raw_buffer_load_b64(rs, vo + 128 * g, 8192 * j, 0));
             }
             fft16k_mfma<+1, P>(v, lds, &ftw, W, t);
             // natural order -> MFMA layout through LDS; the twiddle of element
@@ -651,12 +658,13 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
             float2 Br[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) Br[j] = tw_hi<-1>(tb.nmp, (uint32_t)((4 * kb + j) * k1));
+            const int rb = mfma_rd_base(t);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const float2 Ad = tw<-1>(tb.nmp, (uint32_t)((mfma_elem(g, 0, t) & 1023) * k1));
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    v[4 * g + j] = cconj(cmul(lds_get(lds, lds_re(mfma_elem(g, j, t))), cmul(Ad, Br[j])));
+                    v[4 * g + j] = cconj(cmul(lds_get(lds, mfma_rd(rb, g, j)), cmul(Ad, Br[j])));
                 }
             }
             fft16k_mfma<+1, P>(v, lds, &ftw, W, t);
@@ -1116,13 +1124,14 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
             float2 Br[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) Br[j] = tw_hi<+1>(tb.np, (uint32_t)((4 * kb + j) * tx));
+            const int rb = mfma_rd_base(t);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int e0 = mfma_elem(g, 0, t);
                 const float2 Ad = tw<+1>(tb.np, (uint32_t)((e0 & 1023) * tx));
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    v[4 * g + j] = cmul(lds_get(lds, lds_re(mfma_elem(g, j, t))), cmul(Ad, Br[j]));
+                    v[4 * g + j] = cmul(lds_get(lds, mfma_rd(rb, g, j)), cmul(Ad, Br[j]));
                 }
             }
             MfmaW<+1, P> W;

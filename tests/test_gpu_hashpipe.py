@@ -81,10 +81,10 @@ def test_plugin_two_groups_match_cabi():
     st.close()
 
 
-@pytest.mark.parametrize("variant,trim_mhz", [("trim6", 150), ("trim8", 102)])
+@pytest.mark.parametrize("variant,trim_mhz", [("trim6", 150), ("trim8", 102), ("trim10", 16), ("trim11", 150)])
 def test_plugin_band_trimmed_build_matches_cabi(variant, trim_mhz):
     """A plugin built for the reference thread's band-trimmed path
-    (-DSDBE_PLUGIN_TRIMMED=1 at SWARM_RATE_FRAC 6 or 8): groups of
+    (-DSDBE_PLUGIN_TRIMMED=1 at SWARM_RATE_FRAC 6, 8, 10 or 11): groups of
     DECIMATION_FACTOR B-frames, the trim read from the status key TRIMDC (the
     reference's trim_from_dc, vdif_inout_gpu_thread.cu:770-784), and
     N_32bit_words_per_chan = 2 * 16384 * 128 * E / 16 (:1012).  Output equals

@@ -219,10 +219,10 @@ int sdbe_stream_state_from_template(const void *template_header32,
  * 8192 (PAYLOAD) or 8224 (FRAMES) bytes.  d_bitmap (optional, device):
  * uint8 [frames][256], bit f of [frame][cid] set when packet (frame, cid, f)
  * arrived (the cc[] flags of vdif_in_databuf.h:112-130); with
- * SDBE_GEOM_8X1024 uint16 [frames][16], bit f of [frame][cid].  At the
- * 64/39 rate a 16-byte aligned d_pkts (every hipMalloc'd buffer) lets stage
- * A read the packets directly; other alignments take the int8-row path
- * (same output).                                                          */
+ * SDBE_GEOM_8X1024 uint16 [frames][16], bit f of [frame][cid].  A 16-byte
+ * aligned d_pkts (every hipMalloc'd buffer) lets the first kernel read the
+ * packets directly (stage A at 64/39, the per-snapshot C2R band-trimmed);
+ * other alignments take the int8-row path (same output).                  */
 int sdbe_process_group(sdbe_handle_t *h, const void *d_pkts, int32_t n_pkts,
                        uint64_t b_zero, void *d_out, int64_t out_pol_stride_bytes,
                        uint8_t *d_bitmap, void *stream);

@@ -203,48 +203,87 @@ __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tab
     __syncthreads();
 }
 
-// Radix sequences (first pass Ns = 1, no twiddles), in place, T threads.
+// Radix sequences, in place, T threads: a first pass of radix 16 with Ns = 1
+// (no twiddles), then fft_tail.
 template <int N, int S, int T>
-__device__ __forceinline__ void fft(float2 *a, const float2 *__restrict__ tab) {
+__device__ __forceinline__ void fft_tail(float2 *a, const float2 *__restrict__ tab) {
     if constexpr (N == 4992) {
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 13, 128, S, T>(a, tab);
         pass<N, 3, 1664, S, T>(a, tab);
     } else if constexpr (N == 1664) {
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 13, 128, S, T>(a, tab);
     } else if constexpr (N == 8320) {
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 5, 128, S, T>(a, tab);
         pass<N, 13, 640, S, T>(a, tab);
     } else if constexpr (N == 18304) {
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 11, 128, S, T>(a, tab);
         pass<N, 13, 1408, S, T>(a, tab);
     } else if constexpr (N == 4096) {
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 16, 16, S, T>(a, tab);
         pass<N, 16, 256, S, T>(a, tab);
     } else if constexpr (N == 8192) {
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 16, 16, S, T>(a, tab);
         pass<N, 16, 256, S, T>(a, tab);
         pass<N, 2, 4096, S, T>(a, tab);
     } else if constexpr (N == 16384) {
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 16, 16, S, T>(a, tab);
         pass<N, 16, 256, S, T>(a, tab);
         pass<N, 4, 4096, S, T>(a, tab);
     } else {
         static_assert(N == 1024, "size");
-        pass<N, 16, 1, S, T>(a, tab);
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 8, 128, S, T>(a, tab);
     }
+}
+
+template <int N, int S, int T>
+__device__ __forceinline__ void fft(float2 *a, const float2 *__restrict__ tab) {
+    pass<N, 16, 1, S, T>(a, tab);
+    fft_tail<N, S, T>(a, tab);
+}
+
+// The first pass of fft<N> reading its input x (N points in global memory)
+// directly: butterfly b takes x[b + (N/16) i] into registers, which the LDS
+// staging of the input would only hand back.  Writes as pass<N, 16, 1>, then a
+// barrier.  The buffer must not be in use.
+template <int N, int S, int T>
+__device__ __forceinline__ void pass1_global(float2 *a, const float2 *__restrict__ x) {
+    constexpr int NB = N / 16, IT = (NB + T - 1) / T;
+    float2 v[IT][16];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b0 = threadIdx.x + it * T;
+        const int b = (NB % T == 0 || it < IT - 1) ? b0 : (b0 < NB ? b0 : NB - 1);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[it][i] = x[b + NB * i];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b = threadIdx.x + it * T;
+        dft_r<16, S>(v[it]);
+        if (NB % T == 0 || it < IT - 1 || b < NB) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[swz16(16 * b + k)] = v[it][k];
+        }
+    }
+    __syncthreads();
+}
+
+// The first pass of fft<N> when N / 16 == T and thread t already holds its
+// butterfly's inputs, elements t + (N/16) i, in v[i].  A barrier precedes the
+// writes (the caller's reads of the buffer finish first) and follows them.
+template <int N, int S, int T>
+__device__ __forceinline__ void pass1_regs(float2 *a, float2 (&v)[16]) {
+    static_assert(N / 16 == T, "one butterfly per thread");
+    dft_r<16, S>(v);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[swz16(16 * (int)threadIdx.x + k)] = v[k];
+    __syncthreads();
 }
 
 // Device tables of a preset (host-filled in FP64, rounded to FP32).
@@ -389,9 +428,8 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
     }
     const int bpl = blockIdx.x / trim::NSEG, g = blockIdx.x - bpl * trim::NSEG, t = threadIdx.x;
     const float2 *z = reinterpret_cast<const float2 *>(T + (size_t)bpl * D * 2 * NCH + (size_t)g * R);
-    for (int m = t; m < NR; m += NT) buf[trim::swz16(m)] = z[m];
-    __syncthreads();
-    trim::fft<NR, -1, NT>(buf, tt.zr);
+    trim::pass1_global<NR, -1, NT>(buf, z);   // the input goes to registers, not through LDS
+    trim::fft_tail<NR, -1, NT>(buf, tt.zr);
     // bins skip .. skip + NP of the R2C, then the C2R pre-processing; the NP / 256
     // values of a thread stay in registers until every read of Z is done
     auto X = [&](int k) {
@@ -431,11 +469,18 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
             s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
         }
     } else {
-        __syncthreads();
+        if constexpr (NP / 16 == NT) {
+            // 6/11, 10/11: wv[q] (element t + NT q) is butterfly t's input q of the
+            // first pass, which runs on the registers
+            trim::pass1_regs<NP, +1, NT>(buf, wv);
+            trim::fft_tail<NP, +1, NT>(buf, tt.zp);
+        } else {
+            __syncthreads();
 #pragma unroll
-        for (int q = 0; q < JT; ++q) buf[trim::swz16(t + NT * q)] = wv[q];
-        __syncthreads();
-        trim::fft<NP, +1, NT>(buf, tt.zp);
+            for (int q = 0; q < JT; ++q) buf[trim::swz16(t + NT * q)] = wv[q];
+            __syncthreads();
+            trim::fft<NP, +1, NT>(buf, tt.zp);
+        }
         const float2 *w = buf;
 #pragma unroll 4
         for (int m = t; m < NP; m += NT) {

@@ -246,6 +246,69 @@ __device__ __forceinline__ void fft(float2 *a, const float2 *__restrict__ tab) {
     fft_tail<N, S, T>(a, tab);
 }
 
+// The inverse (C2R) transform sizes split as first pass, fft_mid, last pass;
+// LastPass gives the last pass's radix and Ns (Ns R = N: butterfly b writes
+// outputs b + Ns k).
+template <int N>
+struct LastPass;
+template <>
+struct LastPass<1024> {
+    static constexpr int R = 8, NS = 128;
+};
+template <>
+struct LastPass<4096> {
+    static constexpr int R = 16, NS = 256;
+};
+template <>
+struct LastPass<8192> {
+    static constexpr int R = 2, NS = 4096;
+};
+template <int N, int S, int T>
+__device__ __forceinline__ void fft_mid(float2 *a, const float2 *__restrict__ tab) {
+    if constexpr (N == 4096) {
+        pass<N, 16, 16, S, T>(a, tab);
+    } else if constexpr (N == 8192) {
+        pass<N, 16, 16, S, T>(a, tab);
+        pass<N, 16, 256, S, T>(a, tab);
+    } else {
+        static_assert(N == 1024, "size");
+        pass<N, 8, 16, S, T>(a, tab);
+    }
+}
+
+// The last pass of the transform with its outputs left in registers: thread t
+// runs butterflies b = t + T it and holds output b + Ns k in x[it][k], i.e.
+// element t + T (it + (Ns / T) k), the elements it stores next.  Reads and
+// twiddles as pass<N, R, NS>; nothing is written to the buffer.
+template <int N, int R, int NS, int S, int T>
+__device__ __forceinline__ void pass_to_regs(const float2 *buf, const float2 *__restrict__ tab,
+                                             float2 (&x)[N / R / T][R]) {
+    constexpr int NB = N / R, IT = NB / T, STRIDE = N / (NS * R);
+    static_assert(NB % T == 0 && NS * R == N && NS % T == 0, "last pass");
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b = threadIdx.x + it * T;
+#pragma unroll
+        for (int i = 0; i < R; ++i) x[it][i] = buf[swz16(b + NB * i)];
+        const int m = (b % NS) * STRIDE;
+        float2 pw[16];
+        pw[1] = tab[m];
+        if constexpr (R > 2) pw[2] = tab[2 * m];
+        if constexpr (R > 4) pw[4] = tab[4 * m];
+        if constexpr (R > 8) pw[8] = tab[8 * m];
+#pragma unroll
+        for (int i = 3; i < R; ++i) {
+            if (i == 4 || i == 8) continue;
+            const int hi = i >= 8 ? 8 : (i >= 4 ? 4 : 2);
+            pw[i] = cmul(pw[hi], pw[i - hi]);
+        }
+#pragma unroll
+        for (int i = 1; i < R; ++i) x[it][i] = cmul(x[it][i], pw[i]);
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) dft_r<R, S>(x[it]);
+}
+
 // The first pass of fft<N> reading its input x (N points in global memory)
 // directly: butterfly b takes x[b + (N/16) i] into registers, which the LDS
 // staging of the input would only hand back.  Writes as pass<N, 16, 1>, then a
@@ -473,19 +536,22 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
             // 6/11, 10/11: wv[q] (element t + NT q) is butterfly t's input q of the
             // first pass, which runs on the registers
             trim::pass1_regs<NP, +1, NT>(buf, wv);
-            trim::fft_tail<NP, +1, NT>(buf, tt.zp);
         } else {
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < JT; ++q) buf[trim::swz16(t + NT * q)] = wv[q];
             __syncthreads();
-            trim::fft<NP, +1, NT>(buf, tt.zp);
+            trim::pass<NP, 16, 1, +1, NT>(buf, tt.zp);
         }
-        const float2 *w = buf;
-#pragma unroll 4
-        for (int m = t; m < NP; m += NT) {
-            const float2 x = cscale(w[trim::swz16(m)], sc);
-            dst[m] = x;
+        trim::fft_mid<NP, +1, NT>(buf, tt.zp);
+        // the last pass leaves element t + NT q in registers; stored in q order
+        constexpr int LR = trim::LastPass<NP>::R, LNS = trim::LastPass<NP>::NS, QS = LNS / NT;
+        float2 o[NP / LR / NT][LR];
+        trim::pass_to_regs<NP, LR, LNS, +1, NT>(buf, tt.zp, o);
+#pragma unroll
+        for (int q = 0; q < JT; ++q) {
+            const float2 x = cscale(o[q % QS][q / QS], sc);
+            dst[t + NT * q] = x;
             s1 += (double)x.x + (double)x.y;
             s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
         }

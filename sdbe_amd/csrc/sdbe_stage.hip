@@ -834,10 +834,12 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
         }
         __syncthreads();
         // 64-point inverse DFT over kh per column: 8 threads x radix 8, two passes
-        // (lanes of a half-wave take different columns: odd row stride, no bank conflicts)
+        // (lanes of a half-wave take different columns: odd row stride, no bank
+        // conflicts).  Thread (col, j = rw) ends holding output rows j + 8 r in
+        // v[r]: the rows it stores below, straight from its registers.
+        float2 v[8];
         {
             const int j = rw;
-            float2 v[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
             dft8<+1>(v);
@@ -850,11 +852,7 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
 #pragma unroll
             for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], w64[8 * r + j]);
             dft8<+1>(v);
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < 8; ++r) sc[col][j + 8 * r] = v[r];
         }
-        __syncthreads();
         // write the tile's 32 slots of every row tx: Bc rows are planar (real
         // plane then imaginary plane, BC_ROW floats each): two 128-byte runs.
         // Buffer stores: one VGPR offset, row tx = 8 it + rw in the scalar offset.
@@ -868,7 +866,7 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
             if (x.valid && !x.dup) {
 #pragma unroll
                 for (int it = 0; it < 8; ++it) {
-                    const float2 y = sc[col][it * 8 + rw];
+                    const float2 y = v[it];   // row 8 it + rw
                     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(y.x), rb, vo, 4 * (it * 8 * 2 * BC_ROW),
                                                           CPOL_NT);
                     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(y.y), rb, vo,
@@ -878,7 +876,8 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
         }
         acc_run += acc;
         dc_run += dc;
-        // the next tile writes sz, which is sc's storage: all waves are done with sc
+        // the next tile writes sz, which is sc's storage: all waves are done
+        // reading sc (the DFT's second pass)
         __syncthreads();
         double *o = part + ((size_t)x.bpl * C1_TILES + x.tile) * 2;
         if (!more || (lt + ilv) / C1_TILES != x.bpl) {   // workgroup-uniform

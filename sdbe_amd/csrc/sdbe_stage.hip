@@ -750,7 +750,7 @@ __device__ __forceinline__ void c1_fetch(const float2 *__restrict__ Zf, const Ta
     cl = tb.c1col[NCH + x.c];
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 5)
 k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__restrict__ part, Tables tb, int nb,
            int tpw, int ilv) {
     // sz (packed-spectrum tile) is dead once Y is formed and sc (first-pass
@@ -798,8 +798,10 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
         }
         __syncthreads();
         if (more) c1_fetch(Zf, tb, lt + ilv, col, rw, cur, z, cm, cl);
-        // Y rows 0..39 (zero above M') + Parseval
+        // Y rows 0..39 (zero above M') + Parseval; this thread's rows r = 8 it + rw
+        // also stay in yv for its Zc rows below
         double acc = 0.0, dc = 0.0;
+        float2 yv[5];
 #pragma unroll
         for (int it = 0; it < 5; ++it) {
             const int r = it * 8 + rw;
@@ -820,30 +822,29 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
                     }
                 }
             }
+            yv[it] = Y;
             sy[col][r] = Y;
         }
         __syncthreads();
-        // Zc rows 0..63
+        // Zc rows kh = 8 it + rw (this thread's Y rows for it < 5, the partner
+        // column's from LDS), which are the inputs of this thread's first radix-8
+        // pass of the 64-point inverse DFT over kh: 8 threads x radix 8, two
+        // passes (lanes of a half-wave take different columns: odd row stride, no
+        // bank conflicts).  Thread (col, j = rw) ends holding output rows j + 8 r
+        // in v[r]: the rows it stores below, straight from its registers.
+        float2 v[8];
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             const int kh = it * 8 + rw;
             const int pr = (c == 0) ? (64 - kh) : (63 - kh);
-            const float2 ck = kh <= 39 ? sy[col][kh] : make_float2(0.f, 0.f);
+            const float2 ck = it < 5 ? yv[it] : make_float2(0.f, 0.f);   // kh <= 39
             const float2 cp = pr <= 39 ? cconj(sy[pcol][pr]) : make_float2(0.f, 0.f);
-            sc[col][kh] = cadd(cadd(ck, cp), mul_si<+1>(cmul(cmul(clc, rowl[kh]), csub(ck, cp))));
+            v[it] = cadd(cadd(ck, cp), mul_si<+1>(cmul(cmul(clc, rowl[kh]), csub(ck, cp))));
         }
-        __syncthreads();
-        // 64-point inverse DFT over kh per column: 8 threads x radix 8, two passes
-        // (lanes of a half-wave take different columns: odd row stride, no bank
-        // conflicts).  Thread (col, j = rw) ends holding output rows j + 8 r in
-        // v[r]: the rows it stores below, straight from its registers.
-        float2 v[8];
         {
             const int j = rw;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) v[r] = sc[col][j + 8 * r];
             dft8<+1>(v);
-            __syncthreads();
+            // sc shares sz's storage, which no thread reads after the barrier above
 #pragma unroll
             for (int r = 0; r < 8; ++r) sc[col][8 * j + r] = v[r];
             __syncthreads();

@@ -238,45 +238,26 @@ template <int S> __device__ __forceinline__ void dft13(float2 *x) {
     }
 }
 
-// exp(S*2*pi*i*b*c/39) for c = 1, 2 and b = 0..12 (cos, |sin|) -- sign applied by S
-__device__ __forceinline__ float2 w39(int b, int c) {
-    const float2 t1[13] = {{1.000000000e+00f, 0.0f}, {9.870502626e-01f, 1.604112809e-01f},
-                           {9.485364419e-01f, 3.166679938e-01f}, {8.854560257e-01f, 4.647231720e-01f},
-                           {7.994427634e-01f, 6.007422642e-01f}, {6.927243535e-01f, 7.212024473e-01f},
-                           {5.680647467e-01f, 8.229838659e-01f}, {4.286925614e-01f, 9.034504346e-01f},
-                           {2.782174639e-01f, 9.605181116e-01f}, {1.205366803e-01f, 9.927088741e-01f},
-                           {-4.026594011e-02f, 9.991889982e-01f}, {-2.000256938e-01f, 9.797906520e-01f},
-                           {-3.546048870e-01f, 9.350162427e-01f}};
-    const float2 t2[13] = {{1.000000000e+00f, 0.0f}, {9.485364419e-01f, 3.166679938e-01f},
-                           {7.994427634e-01f, 6.007422642e-01f}, {5.680647467e-01f, 8.229838659e-01f},
-                           {2.782174639e-01f, 9.605181116e-01f}, {-4.026594011e-02f, 9.991889982e-01f},
-                           {-3.546048870e-01f, 9.350162427e-01f}, {-6.324453756e-01f, 7.746049618e-01f},
-                           {-8.451900855e-01f, 5.344658261e-01f}, {-9.709418174e-01f, 2.393156643e-01f},
-                           {-9.967573081e-01f, -8.046656872e-02f}, {-9.199794437e-01f, -3.919666099e-01f},
-                           {-7.485107482e-01f, -6.631226582e-01f}};
-    return c == 1 ? t1[b] : t2[b];
-}
-
-// DFT-39 = 3 x 13: s = 13a + b, k = c + 3d.  In place; output in natural order.
+// DFT-39 by the prime-factor (Good-Thomas) mapping, 39 = 3 x 13 with 3 and 13
+// coprime: input n = (13 n1 + 3 n2) mod 39, output k = (13 k1 + 27 k2) mod 39
+// (13 = 1 mod 3 and 0 mod 13, 27 = 0 mod 3 and 1 mod 13), so that
+// e^{S 2 pi i n k/39} = e^{S 2 pi i n1 k1/3} e^{S 2 pi i n2 k2/13}: three 13-point
+// DFTs over n2, then 13 three-point DFTs over n1, and no twiddle factors.  In
+// place; output in natural order.
 template <int S> __device__ __forceinline__ void dft39(float2 *x) {
+    float2 y[3][13];
 #pragma unroll
-    for (int b = 0; b < 13; ++b) dft3<S>(x[b], x[13 + b], x[26 + b]);
+    for (int n1 = 0; n1 < 3; ++n1)
 #pragma unroll
-    for (int c = 1; c < 3; ++c) {
+        for (int n2 = 0; n2 < 13; ++n2) y[n1][n2] = x[(13 * n1 + 3 * n2) % 39];
 #pragma unroll
-        for (int b = 1; b < 13; ++b) {
-            float2 w = w39(b, c);
-            x[13 * c + b] = cmulc<S>(x[13 * c + b], w.x, w.y);
-        }
+    for (int n1 = 0; n1 < 3; ++n1) dft13<S>(y[n1]);
+#pragma unroll
+    for (int k2 = 0; k2 < 13; ++k2) {
+        dft3<S>(y[0][k2], y[1][k2], y[2][k2]);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) x[(13 * k1 + 27 * k2) % 39] = y[k1][k2];
     }
-    dft13<S>(x);
-    dft13<S>(x + 13);
-    dft13<S>(x + 26);
-    float2 t[39];
-#pragma unroll
-    for (int k = 0; k < 39; ++k) t[k] = x[13 * (k % 3) + k / 3];
-#pragma unroll
-    for (int k = 0; k < 39; ++k) x[k] = t[k];
 }
 
 // Workgroup `lin` of a 1D grid of `total` runs on XCD lin % 8 (round-robin

@@ -279,11 +279,40 @@ constexpr int CPOL_NT = 2;
 // packet path passes 2-bit codes (value + 2) and folds the -2 offsets into the
 // per-lane constants cs, cd; the int8 path passes values and zero constants.
 // Both give exactly the same float inputs to the first rounding, cmul(T, dif).
-__device__ __forceinline__ float2 a_input(float Ra, float Ia, float Rb, float Ib, float kz, float2 cs, float2 cd,
-                                          float2 T) {
-    const float2 sum = make_float2(fmaf(kz, Rb, Ra) + cs.x, kz * (Ia - Ib));          // DC imaginary dropped,
-    const float2 dif = make_float2(fmaf(-kz, Rb, Ra) + cd.x, fmaf(kz, Ia + Ib, cd.y));  // Nyquist bin zero
-    return cadd(sum, mul_si<+1>(cmul(T, dif)));
+// Z_s[k] = sum + i T dif from the channel pair's values X[k] = (Ra, Ia) and
+// X[N2-k] = (Rb, Ib) (offsets removed), combined as integers first:
+// sum = X[k] + conj X[N2-k] = (sx, sy), dif = X[k] - conj X[N2-k] = (dx, dy).
+// For k = 0 the DC imaginary part is dropped and the Nyquist bin is zero (as
+// numpy irfft): sum = dif = (Ra, 0).
+__device__ __forceinline__ float2 a_combine(int sx, int sy, int dx, int dy, float2 T) {
+    const float fx = (float)dx, fy = (float)dy;
+    return make_float2(fmaf(-T.x, fy, fmaf(-T.y, fx, (float)sx)), fmaf(T.x, fx, fmaf(-T.y, fy, (float)sy)));
+}
+
+// The 39 snapshots' Z_s[k] from 2-bit codes (value = code - 2): code words wa
+// (channel k) and wb (channel N2 - k) of snapshot s at word(s), field (im at bit
+// sh, re at bit sh + 2).  k = 0 (one thread of the block-pol) takes its own loop.
+template <typename Word>
+__device__ __forceinline__ void a_inputs_codes(float2 (&x)[NBLK], int k, uint32_t sha, uint32_t shb, float2 T,
+                                               Word word) {
+    if (k != 0) {
+#pragma unroll
+        for (int s = 0; s < NBLK; ++s) {
+            uint32_t wa, wb;
+            word(s, wa, wb);
+            const int Ia = (int)__builtin_amdgcn_ubfe(wa, sha, 2u), Ra = (int)__builtin_amdgcn_ubfe(wa, sha + 2u, 2u);
+            const int Ib = (int)__builtin_amdgcn_ubfe(wb, shb, 2u), Rb = (int)__builtin_amdgcn_ubfe(wb, shb + 2u, 2u);
+            x[s] = a_combine(Ra + Rb - 4, Ia - Ib, Ra - Rb, Ia + Ib - 4, T);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NBLK; ++s) {
+            uint32_t wa, wb;
+            word(s, wa, wb);
+            const int Ra = (int)__builtin_amdgcn_ubfe(wa, sha + 2u, 2u) - 2;
+            x[s] = a_combine(Ra, 0, Ra, 0, T);
+        }
+    }
 }
 
 typedef unsigned int u32v2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
@@ -325,15 +354,22 @@ k_stage_a(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, float2
     const int flip = bpd[bpl].flip;
     const int km = (NCH - k) & (NCH - 1);      // k = 0 pairs with the zero Nyquist bin
     const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
-    const float kz = k == 0 ? 0.f : 1.f;
     const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
     float2 x[NBLK];
+    if (k != 0) {
 #pragma unroll
-    for (int s = 0; s < NBLK; ++s) {
-        const uint16_t ua = src[(size_t)s * NCH + ka], ub = src[(size_t)s * NCH + kb];
-        const float Ia = (float)(int8_t)(ua & 0xFF), Ra = (float)(int8_t)(ua >> 8);   // (im, re) values
-        const float Ib = (float)(int8_t)(ub & 0xFF), Rb = (float)(int8_t)(ub >> 8);
-        x[s] = a_input(Ra, Ia, Rb, Ib, kz, make_float2(0.f, 0.f), make_float2(0.f, 0.f), T);
+        for (int s = 0; s < NBLK; ++s) {
+            const uint16_t ua = src[(size_t)s * NCH + ka], ub = src[(size_t)s * NCH + kb];
+            const int Ia = (int8_t)(ua & 0xFF), Ra = (int8_t)(ua >> 8);   // (im, re) values
+            const int Ib = (int8_t)(ub & 0xFF), Rb = (int8_t)(ub >> 8);
+            x[s] = a_combine(Ra + Rb, Ia - Ib, Ra - Rb, Ia + Ib, T);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NBLK; ++s) {
+            const int Ra = (int8_t)(src[(size_t)s * NCH + ka] >> 8);
+            x[s] = a_combine(Ra, 0, Ra, 0, T);
+        }
     }
     dft39<-1>(x);
     a_store(V, Zf, bpl, k, x);
@@ -405,21 +441,13 @@ k_stage_a_pk(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, flo
     const int ob = 4 * APK_US * (APK_GA + sb) + off + ((kb & 7) >> 2);
     const int pbit = pol == 0 ? 4 : 0;
     const int sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
-    const float kz = k == 0 ? 0.f : 1.f;
     const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
-    // the codes' -2 offsets in sum and dif
-    const float2 cs = make_float2(-2.f - 2.f * kz, 0.f), cd = make_float2(-2.f + 2.f * kz, -4.f * kz);
     float2 x[NBLK];
-#pragma unroll
-    for (int s = 0; s < NBLK; ++s) {
-        const uint32_t wa = seg32[oa + 2 * s], wb = seg32[ob + 2 * s];
-        // codes: im in the nibble's bits 0-1, re in bits 2-3
-        const float Ia = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha, 2u);
-        const float Ra = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha + 2u, 2u);
-        const float Ib = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb, 2u);
-        const float Rb = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb + 2u, 2u);
-        x[s] = a_input(Ra, Ia, Rb, Ib, kz, cs, cd, T);
-    }
+    // codes: im in the nibble's bits 0-1, re in bits 2-3
+    a_inputs_codes(x, k, (uint32_t)sha, (uint32_t)shb, T, [&](int s, uint32_t &wa, uint32_t &wb) {
+        wa = seg32[oa + 2 * s];
+        wb = seg32[ob + 2 * s];
+    });
     dft39<-1>(x);
     a_store(V, Zf, bpl, k, x);
 }
@@ -482,20 +510,12 @@ k_stage_a_pk8(const BpDesc *__restrict__ bpd, int nb, float2 *__restrict__ V, fl
     const int oa = (ka >> 2) - wa0, ob = 4 * AP8_QA + min(max((kb >> 2) - wb0, 0), 4 * AP8_QB - 1);
     const int pbit = pol == 0 ? 4 : 0;
     const int sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
-    const float kz = k == 0 ? 0.f : 1.f;
     const float2 T = tw<+1>(tb.n32k, (uint32_t)k);
-    // the codes' -2 offsets in sum and dif
-    const float2 cs = make_float2(-2.f - 2.f * kz, 0.f), cd = make_float2(-2.f + 2.f * kz, -4.f * kz);
     float2 x[NBLK];
-#pragma unroll
-    for (int s = 0; s < NBLK; ++s) {
-        const uint32_t wa = seg32[AP8_W * s + oa], wb = seg32[AP8_W * s + ob];
-        const float Ia = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha, 2u);
-        const float Ra = (float)__builtin_amdgcn_ubfe(wa, (uint32_t)sha + 2u, 2u);
-        const float Ib = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb, 2u);
-        const float Rb = (float)__builtin_amdgcn_ubfe(wb, (uint32_t)shb + 2u, 2u);
-        x[s] = a_input(Ra, Ia, Rb, Ib, kz, cs, cd, T);
-    }
+    a_inputs_codes(x, k, (uint32_t)sha, (uint32_t)shb, T, [&](int s, uint32_t &wa, uint32_t &wb) {
+        wa = seg32[AP8_W * s + oa];
+        wb = seg32[AP8_W * s + ob];
+    });
     dft39<-1>(x);
     a_store(V, Zf, bpl, k, x);
 }
@@ -682,11 +702,14 @@ k_stage_b(const float2 *__restrict__ V, float2 *__restrict__ Zf, Tables tb, int 
 
 // Stage C1: workgroup = 16 column pairs (c, 16384 - c) of the [64][16384] view
 // (k = c + 16384 kh) of the packed 2^20-point C2R input, all 64 rows.
-//   Y[m]  = 1/2 (Zf[m] + conj Zf[M'-m]) - 1/2 i e^{-2 pi i m/M} (Zf[m] - conj Zf[M'-m]),  m <= M'
+//   Y[m]  = (Zf[m] + conj Zf[M'-m]) - i e^{-2 pi i m/M} (Zf[m] - conj Zf[M'-m]),  m <= M'
+// (twice the R2C spectrum: the factor 1/2 is left to the statistics and stage C2's
+// output scale, both exact powers of two)
 //   Zc[k] = (C[k] + conj C[P-k]) + i e^{2 pi i k/L} (C[k] - conj C[P-k]),  C = Y zero-padded
 //   Bc[tx][c] = sum_kh Zc[c + 16384 kh] e^{2 pi i kh tx/64}
 // (the inter-pass twiddle e^{2 pi i c tx/P} is applied by stage C2 on load)
-// and the FP64 Parseval sum  sum_m w_m |Y[m]|^2 (w_0 = 1 on Re Y[0], else 2).
+// and the Parseval sum  sum_m w_m |Y[m]|^2 (w_0 = 1 on Re Y[0], else 2): five-term
+// FP32 partials per thread, accumulated in FP64.
 // Tile (a, u) takes the lower columns c = a + 39 (16 u + j), j < 16: for each
 // row r the 16 values m = c + 16384 r share the packed-spectrum row
 // (m mod 39 = (a + 4 r) mod 39) at consecutive k2, so every read is a
@@ -799,8 +822,11 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
         __syncthreads();
         if (more) c1_fetch(Zf, tb, lt + ilv, col, rw, cur, z, cm, cl);
         // Y rows 0..39 (zero above M') + Parseval; this thread's rows r = 8 it + rw
-        // also stay in yv for its Zc rows below
-        double acc = 0.0, dc = 0.0;
+        // also stay in yv for its Zc rows below.  The Parseval terms of its five
+        // rows are summed in FP32 (relative error ~2^-22 on a five-term partial,
+        // ~1e-10 on the block's sum) before the FP64 accumulation across tiles.
+        float accf = 0.f;
+        double dc = 0.0;
         float2 yv[5];
 #pragma unroll
         for (int it = 0; it < 5; ++it) {
@@ -810,15 +836,13 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
                 const int pr = (c == 0) ? (39 - r) : (38 - r);
                 const float2 zm = sz[col][r];
                 const float2 zp = cconj(sz[pcol][pr]);
-                const float2 E = cscale(cadd(zm, zp), 0.5f);
-                const float2 D = cscale(csub(zm, zp), 0.5f);
-                Y = cadd(E, mul_si<-1>(cmul(cmul(cmc, rowm[r]), D)));
+                Y = cadd(cadd(zm, zp), mul_si<-1>(cmul(cmul(cmc, rowm[r]), csub(zm, zp))));
                 if (!x.dup) {
                     if (c == 0 && r == 0) {
-                        acc += (double)Y.x * (double)Y.x;
+                        accf += 0.5f * Y.x * Y.x;   // weight 1 of 2 (the factor 2 is applied below)
                         dc = (double)Y.x;
                     } else {
-                        acc += 2.0 * ((double)Y.x * (double)Y.x + (double)Y.y * (double)Y.y);
+                        accf += Y.x * Y.x + Y.y * Y.y;
                     }
                 }
             }
@@ -875,7 +899,7 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
                 }
             }
         }
-        acc_run += acc;
+        acc_run += 2.0 * (double)accf;
         dc_run += dc;
         // the next tile writes sz, which is sc's storage: all waves are done
         // reading sc (the DFT's second pass)
@@ -907,8 +931,8 @@ k_stage_c1(const float2 *__restrict__ Zf, float *__restrict__ Bc, double *__rest
 }
 
 // Statistics and thresholds of a chunk.  Units: the pipeline's samples are
-// 32768 * 2^21 = 2^36 times numpy's; Y here carries a factor 32768.
-//   sum x^2 = W / (32768^2 * L),  mean = Re Y[0] / (32768 * L),
+// 65536 * 2^21 = 2^37 times numpy's; Y here carries a factor 65536.
+//   sum x^2 = W / (65536^2 * L),  mean = Re Y[0] / (65536 * L),
 //   sigma = sqrt((sum x^2 - L mean^2) / (L - 1))
 // PER_BLOCK and FIXED thresholds depend on the block-pol alone and are written
 // here; the REFERENCE walk (k_thresh) runs only for chunks that need it.
@@ -917,7 +941,7 @@ k_stats(const double *__restrict__ part, int bp0, double2 *__restrict__ ms, doub
         const BpDesc *__restrict__ bpd, float2 *__restrict__ thr) {
     __shared__ double red[2][256];
     const int t = threadIdx.x, b = blockIdx.x;
-    const double L = 2097152.0, FA = 32768.0;
+    const double L = 2097152.0, FA = 65536.0;
     double a = 0.0, d = 0.0;
     for (int i = t; i < C1_TILES; i += 256) {
         a += part[((size_t)b * C1_TILES + i) * 2];
@@ -1015,7 +1039,7 @@ struct OutDesc {
 
 // Stage C2: row (tx, block-pol), row index KHI bpl + tx.
 // z[tx + 64 ty] = sum_c Bc[tx][c] e^{2 pi i c tx/2^20} e^{2 pi i c ty/16384} is the packed output:
-// samples (2t, 2t+1) = (Re, Im) z[t], scaled by 2^-36 to numpy units.  Codes
+// samples (2t, 2t+1) = (Re, Im) z[t], scaled by 2^-37 to numpy units.  Codes
 // per vdif_inout_gpu_thread.cu:577 (v = x - off; ((|v| >= th) | 2) ^ 3 (v < 0));
 // the 4-bit pair code (re | im << 2) of t goes to the low nibble of
 // stage[bp][tx / 2][ty] for even tx, the high nibble for odd tx.
@@ -1065,7 +1089,7 @@ k_stage_c2(const float *__restrict__ Bc, const float2 *__restrict__ thr, const B
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, tb.fft);
     const int t = threadIdx.x;
-    const float scale = 1.4551915228366852e-11f;   // 2^-36, exact
+    const float scale = 7.275957614183426e-12f;   // 2^-37, exact
     // LDS word of each slot of this thread, two 16-bit words per register
     // (slot_col has C2_SLOT_TAB entries, -1 past the row: all loads are issued
     // before any is used)

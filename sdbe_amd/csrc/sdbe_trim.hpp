@@ -76,6 +76,12 @@ inline bool geometry(int rate_frac, int skip_mhz, Geometry &g) {
 // group; with the swizzle every read and write of the passes spreads over the
 // 16 groups (N is a multiple of 16 for every size here).
 __device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
+// swz16(e + c) from sb = swz16(e) for a constant c that is a multiple of 128:
+// adding c leaves bits 0-3 of e alone and adds 0 or 8 (c mod 256 = 0 or 128)
+// to bits 4-7 modulo 16, an XOR with 8, so swz16(e + c) = (sb ^ 8 [c mod 256 =
+// 128]) + c.  With c known at compile time the access is one of at most two
+// base registers plus an immediate offset.
+__device__ __forceinline__ int swz_add(int sb, int c) { return (sb ^ ((c & 128) >> 4)) + c; }
 
 // DFT of odd prime length P by conjugate pairing, in place, sign S:
 // X[d] = x0 + A_d + S i B_d and X[P-d] = x0 + A_d - S i B_d with
@@ -159,19 +165,58 @@ __device__ __forceinline__ void dft_r(float2 *x) {
 // powers are products of at most four) is issued before the first is used.
 // Loads of a partial last iteration read butterfly N/R - 1; only its writes are
 // guarded.  In place: all reads of the pass happen before the barrier that
-// precedes the first write.
-template <int N, int R, int NS, int S, int T>
+// precedes the first write.  The buffer is swizzled (swz16) on input and on
+// output, or plain on output with LIN (a last pass: with Ns >= 64 its writes
+// are contiguous over a wave).  Every stride N/R of the reads here is a multiple
+// of 128 (swz_add); the writes' per-output offsets are immediate for Ns a
+// multiple of 128 (swz_add) and an XOR with a constant for Ns = 1 and 16.
+template <int N, int R, int S, int T>
+__device__ __forceinline__ void pass_reads(const float2 *in, int b, float2 (&x)[R]) {
+    constexpr int NB = N / R;
+    if constexpr (NB % 128 == 0) {
+        const int sb = swz16(b);
+#pragma unroll
+        for (int i = 0; i < R; ++i) x[i] = in[swz_add(sb, NB * i)];
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) x[i] = in[swz16(b + NB * i)];
+    }
+}
+template <int N, int R, int NS, bool LIN>
+__device__ __forceinline__ void pass_writes(float2 *out, int b, const float2 (&x)[R]) {
+    const int base = (b / NS) * NS * R + (b % NS);
+    if constexpr (LIN) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[base + NS * k] = x[k];
+    } else if constexpr (NS % 128 == 0) {
+        const int sb = swz16(base);
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[swz_add(sb, NS * k)] = x[k];
+    } else if constexpr (NS == 16 && (R & (R - 1)) == 0) {
+        // bits 4 .. 3 + log2 R of base are zero, so adding 16 k (k < R) sets them
+        // without a carry: swz16(base + 16 k) = (swz16(base) ^ k) + 16 k
+        const int sb = swz16(base);
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[(sb ^ k) + 16 * k] = x[k];
+    } else if constexpr (NS == 1 && R == 16) {
+        const int sb = swz16(base);   // base = 16 b: swz16(16 b + k) = swz16(16 b) ^ k
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[sb ^ k] = x[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[swz16(base + NS * k)] = x[k];
+    }
+}
+template <int N, int R, int NS, int S, int T, bool LIN = false>
 __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tab) {
-    const float2 *in = buf;
-    float2 *out = buf;
     constexpr int NB = N / R, IT = (NB + T - 1) / T, STRIDE = N / (NS * R);
+    static_assert(!LIN || NS >= 64, "plain output layout needs contiguous writes");
     float2 x[IT][R];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int b0 = threadIdx.x + it * T;
         const int b = (NB % T == 0 || it < IT - 1) ? b0 : (b0 < NB ? b0 : NB - 1);
-#pragma unroll
-        for (int i = 0; i < R; ++i) x[it][i] = in[swz16(b + NB * i)];
+        pass_reads<N, R, S, T>(buf, b, x[it]);
         if constexpr (NS > 1) {
             const int m = (b % NS) * STRIDE;
             float2 pw[16];
@@ -194,56 +239,33 @@ __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tab
     for (int it = 0; it < IT; ++it) {
         const int b = threadIdx.x + it * T;
         dft_r<R, S>(x[it]);
-        if (NB % T == 0 || it < IT - 1 || b < NB) {
-            const int base = (b / NS) * NS * R + (b % NS);
-#pragma unroll
-            for (int k = 0; k < R; ++k) out[swz16(base + NS * k)] = x[it][k];
-        }
+        if (NB % T == 0 || it < IT - 1 || b < NB) pass_writes<N, R, NS, LIN>(buf, b, x[it]);
     }
     __syncthreads();
 }
 
-// Radix sequences, in place, T threads: a first pass of radix 16 with Ns = 1
-// (no twiddles), then fft_tail.
+// The R2C transform sizes after their first pass (pass1_global): radix passes in
+// place, T threads.  The last pass writes the plain layout (element e at e),
+// which the post-processing reads.
 template <int N, int S, int T>
 __device__ __forceinline__ void fft_tail(float2 *a, const float2 *__restrict__ tab) {
     if constexpr (N == 4992) {
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 13, 128, S, T>(a, tab);
-        pass<N, 3, 1664, S, T>(a, tab);
+        pass<N, 3, 1664, S, T, true>(a, tab);
     } else if constexpr (N == 1664) {
         pass<N, 8, 16, S, T>(a, tab);
-        pass<N, 13, 128, S, T>(a, tab);
+        pass<N, 13, 128, S, T, true>(a, tab);
     } else if constexpr (N == 8320) {
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 5, 128, S, T>(a, tab);
-        pass<N, 13, 640, S, T>(a, tab);
-    } else if constexpr (N == 18304) {
+        pass<N, 13, 640, S, T, true>(a, tab);
+    } else {
+        static_assert(N == 18304, "size");
         pass<N, 8, 16, S, T>(a, tab);
         pass<N, 11, 128, S, T>(a, tab);
-        pass<N, 13, 1408, S, T>(a, tab);
-    } else if constexpr (N == 4096) {
-        pass<N, 16, 16, S, T>(a, tab);
-        pass<N, 16, 256, S, T>(a, tab);
-    } else if constexpr (N == 8192) {
-        pass<N, 16, 16, S, T>(a, tab);
-        pass<N, 16, 256, S, T>(a, tab);
-        pass<N, 2, 4096, S, T>(a, tab);
-    } else if constexpr (N == 16384) {
-        pass<N, 16, 16, S, T>(a, tab);
-        pass<N, 16, 256, S, T>(a, tab);
-        pass<N, 4, 4096, S, T>(a, tab);
-    } else {
-        static_assert(N == 1024, "size");
-        pass<N, 8, 16, S, T>(a, tab);
-        pass<N, 8, 128, S, T>(a, tab);
+        pass<N, 13, 1408, S, T, true>(a, tab);
     }
-}
-
-template <int N, int S, int T>
-__device__ __forceinline__ void fft(float2 *a, const float2 *__restrict__ tab) {
-    pass<N, 16, 1, S, T>(a, tab);
-    fft_tail<N, S, T>(a, tab);
 }
 
 // The inverse (C2R) transform sizes split as first pass, fft_mid, last pass;
@@ -288,8 +310,7 @@ __device__ __forceinline__ void pass_to_regs(const float2 *buf, const float2 *__
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int b = threadIdx.x + it * T;
-#pragma unroll
-        for (int i = 0; i < R; ++i) x[it][i] = buf[swz16(b + NB * i)];
+        pass_reads<N, R, S, T>(buf, b, x[it]);
         const int m = (b % NS) * STRIDE;
         float2 pw[16];
         pw[1] = tab[m];
@@ -328,10 +349,7 @@ __device__ __forceinline__ void pass1_global(float2 *a, const float2 *__restrict
     for (int it = 0; it < IT; ++it) {
         const int b = threadIdx.x + it * T;
         dft_r<16, S>(v[it]);
-        if (NB % T == 0 || it < IT - 1 || b < NB) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) a[swz16(16 * b + k)] = v[it][k];
-        }
+        if (NB % T == 0 || it < IT - 1 || b < NB) pass_writes<N, 16, 1, false>(a, b, v[it]);
     }
     __syncthreads();
 }
@@ -344,8 +362,7 @@ __device__ __forceinline__ void pass1_regs(float2 *a, float2 (&v)[16]) {
     static_assert(N / 16 == T, "one butterfly per thread");
     dft_r<16, S>(v);
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) a[swz16(16 * (int)threadIdx.x + k)] = v[k];
+    pass_writes<N, 16, 1, false>(a, (int)threadIdx.x, v);
     __syncthreads();
 }
 
@@ -492,33 +509,57 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
     const int bpl = blockIdx.x / trim::NSEG, g = blockIdx.x - bpl * trim::NSEG, t = threadIdx.x;
     const float2 *z = reinterpret_cast<const float2 *>(T + (size_t)bpl * D * 2 * NCH + (size_t)g * R);
     trim::pass1_global<NR, -1, NT>(buf, z);   // the input goes to registers, not through LDS
-    trim::fft_tail<NR, -1, NT>(buf, tt.zr);
+    trim::fft_tail<NR, -1, NT>(buf, tt.zr);   // Z in natural order, plain layout
     // bins skip .. skip + NP of the R2C, then the C2R pre-processing; the NP / 256
-    // values of a thread stay in registers until every read of Z is done
-    auto X = [&](int k) {
-        const float2 a = buf[trim::swz16(k == NR ? 0 : k)], b = cconj(buf[trim::swz16((NR - k) % NR)]);
-        const float2 e = cscale(cadd(a, b), 0.5f), d = cscale(csub(a, b), 0.5f);
-        return cadd(e, mul_si<-1>(cmul(tt.xr[k], d)));
+    // values of a thread stay in registers until every read of Z is done.
+    // 2 X[k] = (Z[k] + conj Z[NR-k]) - i xr[k] (Z[k] - conj Z[NR-k]) (twice the
+    // R2C spectrum: the factor 1/2 goes into the output scale, a power of two).
+    // Element j = t + NT q needs k = skip + j and k' = skip + NP - j: the reads
+    // Z[k], Z[NR-k], Z[k'], Z[NR-k'] and the table entries xr[k], xr[k'], wp[j]
+    // are per-thread bases plus multiples of NT.  Only j = 0 (t = 0, q = 0) meets
+    // the ends: Z[NR-k] with k = 0 and Z[k'] with k' = NR both read Z[0].
+    auto X2 = [&](int ia, int ib, float2 w) {
+        const float2 a = buf[ia], b = cconj(buf[ib]);
+        return cadd(cadd(a, b), mul_si<-1>(cmul(w, csub(a, b))));
     };
     constexpr int JT = NP / NT;
     static_assert(NP % NT == 0, "C2R size");
+    // k = ka + NT q, NR - k = kb + NT (JT - 1 - q), k' = kc + NT (JT - 1 - q),
+    // NR - k' = kd + NT q: every offset from a base is a non-negative constant
+    const int ka = skip + t, kc = skip + NT - t;
+    const int kb = NR - NT * (JT - 1) - ka, kd = NR - NT * (JT - 1) - kc;
+    const __amdgpu_buffer_rsrc_t rxr =
+        __builtin_amdgcn_make_buffer_rsrc((void *)tt.xr, 0, (NR + 1) * (int)sizeof(float2), BUF_RSRC_WORD3);
+    const __amdgpu_buffer_rsrc_t rwp =
+        __builtin_amdgcn_make_buffer_rsrc((void *)tt.wp, 0, NP * (int)sizeof(float2), BUF_RSRC_WORD3);
+    auto ld = [](__amdgpu_buffer_rsrc_t r, int vo, int so) {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    };
     float2 wv[JT];
 #pragma unroll
     for (int q = 0; q < JT; ++q) {
-        const int j = t + NT * q;
-        float2 y = X(skip + j), ym = X(skip + NP - j);
-        if (j == 0) {
+        const int qd = JT - 1 - q;
+        int ib = kb + NT * qd, ic = kc + NT * qd;
+        if (q == 0) {
+            ib = ib == NR ? 0 : ib;
+            ic = ic == NR ? 0 : ic;
+        }
+        float2 y = X2(ka + NT * q, ib, ld(rxr, 8 * ka, 8 * NT * q));
+        float2 ym = X2(ic, kd + NT * q, ld(rxr, 8 * kc, 8 * NT * qd));
+        if (q == 0 && t == 0) {
             y.y = 0.f;         // Y[0] and Y[P/2] enter with their real parts only
             ym.y = 0.f;
         }
         const float2 sum = cadd(y, cconj(ym)), dif = csub(y, cconj(ym));
-        wv[q] = cadd(sum, mul_si<+1>(cmul(tt.wp[j], dif)));
+        wv[q] = cadd(sum, mul_si<+1>(cmul(ld(rwp, 8 * t, 8 * NT * q), dif)));
     }
-    // samples (2m, 2m+1) = (Re, Im) w[m] / (32768 P)
-    const float sc = 1.0f / (32768.0f * (float)P);
+    // samples (2m, 2m+1) = (Re, Im) w[m] / (65536 P)
+    const float sc = 1.0f / (65536.0f * (float)P);
     float2 *dst = reinterpret_cast<float2 *>((F32 ? (float *)bpd[bpl].out : stage + (size_t)bpl * trim::NSEG * P) +
                                              (size_t)g * P);
-    double s1 = 0.0, s2 = 0.0;
+    // the segment's sums of x and x^2: FP32 over the thread's 2 JT samples,
+    // then FP64 across threads and segments
+    float s1f = 0.f, s2f = 0.f;
     if constexpr (F16K) {
         // wv[q] is element t + 1024 q, fft16k_t's input layout; it waits at its
         // first barrier until every read of Z is done
@@ -528,8 +569,8 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
         for (int q = 0; q < JT; ++q) {
             const float2 x = cscale(wv[q], sc);
             d[1024 * q] = x;
-            s1 += (double)x.x + (double)x.y;
-            s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+            s1f += x.x + x.y;
+            s2f = fmaf(x.x, x.x, fmaf(x.y, x.y, s2f));
         }
     } else {
         if constexpr (NP / 16 == NT) {
@@ -537,25 +578,31 @@ k_trim_seg(const float *__restrict__ T, const BpDesc *__restrict__ bpd, float *_
             // first pass, which runs on the registers
             trim::pass1_regs<NP, +1, NT>(buf, wv);
         } else {
+            static_assert(NT % 128 == 0, "swz_add offsets");
             __syncthreads();
+            const int sb = trim::swz16(t);
 #pragma unroll
-            for (int q = 0; q < JT; ++q) buf[trim::swz16(t + NT * q)] = wv[q];
+            for (int q = 0; q < JT; ++q) buf[trim::swz_add(sb, NT * q)] = wv[q];
             __syncthreads();
             trim::pass<NP, 16, 1, +1, NT>(buf, tt.zp);
         }
         trim::fft_mid<NP, +1, NT>(buf, tt.zp);
         // the last pass leaves element t + NT q in registers; stored in q order
+        // through a buffer resource (the q offsets in the scalar operand)
         constexpr int LR = trim::LastPass<NP>::R, LNS = trim::LastPass<NP>::NS, QS = LNS / NT;
         float2 o[NP / LR / NT][LR];
         trim::pass_to_regs<NP, LR, LNS, +1, NT>(buf, tt.zp, o);
+        const __amdgpu_buffer_rsrc_t rd =
+            __builtin_amdgcn_make_buffer_rsrc((void *)dst, 0, NP * (int)sizeof(float2), BUF_RSRC_WORD3);
 #pragma unroll
         for (int q = 0; q < JT; ++q) {
             const float2 x = cscale(o[q % QS][q / QS], sc);
-            dst[t + NT * q] = x;
-            s1 += (double)x.x + (double)x.y;
-            s2 += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32v2, x), rd, 8 * t, 8 * NT * q, 0);
+            s1f += x.x + x.y;
+            s2f = fmaf(x.x, x.x, fmaf(x.y, x.y, s2f));
         }
     }
+    double s1 = (double)s1f, s2 = (double)s2f;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         s1 += __shfl_xor(s1, off);

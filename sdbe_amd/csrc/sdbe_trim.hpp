@@ -22,8 +22,9 @@
 //   11/11 (D 143, E 128): 18304 = 16 8 11 13, 16384 = 16 16 16 4
 //
 // Units: the unnormalised chain is numpy's irfft / rfft / irfft times
-// 32768 P; k_trim_seg removes that exact power of two, so samples, statistics
-// and thresholds are in numpy units.
+// 32768 P, and 65536 P inside k_trim_seg, whose R2C post-processing leaves out
+// its factor 1/2; k_trim_seg removes that exact power of two, so samples,
+// statistics and thresholds are in numpy units.
 #pragma once
 
 namespace trim {

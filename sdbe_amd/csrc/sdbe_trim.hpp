@@ -375,6 +375,35 @@ struct Tabs {
     const float2 *zp;     // e^{+2 pi i m/(P/2)}, m < P/2         (inverse P/2-point FFT)
 };
 
+// Low-index bits of the N = 32768 two-level twiddle table (tw<S>(tab, k) =
+// hi[k >> 7] lo[k & 127]).
+constexpr int N32K_LO = 7;
+
+// The snapshot C2R's input elements k = t + 1024 q (q < 16) of thread t, for
+// k_trim_c2r and k_trim_c2r_pk alike (so that both give the same bits).  From
+// the values (offsets removed) of channel k, (Ra, Ia), and of its mirror N2 - k,
+// (Rb, Ib), given by val(q, Ra, Ia, Rb, Ib): Z[k] = sum + i e^{2 pi i k/32768}
+// dif (a_combine: the sums and differences as integers first), and for k = 0
+// the DC real part only (the Nyquist bin is zero).  The twiddle is the product
+// tw<+1>(tb.n32k, k) forms, hi[k >> 7] lo[k & 127]: lo[t & 127] is the same
+// for every q, and hi[(t >> 7) + 8 q] for the 64 threads of a wave, which read
+// it through the scalar cache.
+template <typename Val>
+__device__ __forceinline__ void c2r_inputs(const Tw2 &n32k, float2 (&v)[16], Val val) {
+    const int t = threadIdx.x;
+    const int hb = __builtin_amdgcn_readfirstlane(t >> N32K_LO);
+    const F2v l = ((const ConstF2 *)n32k.lo)[t & ((1 << N32K_LO) - 1)];
+    const float2 lo = make_float2(l.x, l.y);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const F2v h = ((const ConstF2 *)n32k.hi)[hb + (1024 >> N32K_LO) * q];
+        const float2 T = cmul(make_float2(h.x, h.y), lo);
+        int Ra, Ia, Rb, Ib;
+        val(q, Ra, Ia, Rb, Ib);
+        v[q] = (q == 0 && t == 0) ? a_combine(Ra, 0, Ra, 0, T) : a_combine(Ra + Rb, Ia - Ib, Ra - Rb, Ia + Ib, T);
+    }
+}
+
 }  // namespace trim
 
 // SwarmC2R of one snapshot (workgroup = snapshot s of block bpl, 1024 threads):
@@ -391,19 +420,20 @@ k_trim_c2r(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb) {
     const int bpl = blockIdx.x / D, s = blockIdx.x - bpl * D, t = threadIdx.x;
     const GAS uint16_t *src = gptr(reinterpret_cast<const uint16_t *>(bpd[bpl].rows)) + (size_t)s * NCH;
     const int flip = bpd[bpl].flip;
+    // channel ka = k (flipped: N - 1 - k) and its mirror kb = N - k (flipped:
+    // k - 1; k = 0 pairs with the zero Nyquist bin and reads nothing used)
+    const int da = flip ? -1024 : 1024;
+    const int ea = flip ? NCH - 1 - t : t, eb = flip ? t - 1 : NCH - t;
     float2 v[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int k = t + 1024 * q;
-        const int km = (NCH - k) & (NCH - 1);
-        const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
-        const float kz = k == 0 ? 0.f : 1.f;
-        const uint16_t ua = src[ka], ub = src[kb];
-        const float2 a = make_float2((float)(int8_t)(ua >> 8), kz * (float)(int8_t)(ua & 0xFF));
-        const float2 b = make_float2(kz * (float)(int8_t)(ub >> 8), kz * (float)(int8_t)(ub & 0xFF));
-        const float2 sum = cadd(a, cconj(b)), dif = csub(a, cconj(b));
-        v[q] = cadd(sum, mul_si<+1>(cmul(tw<+1>(tb.n32k, (uint32_t)k), dif)));
-    }
+    trim::c2r_inputs(tb.n32k, v, [&](int q, int &Ra, int &Ia, int &Rb, int &Ib) {
+        // only k = 0 (t = 0, q = 0) has its mirror outside the row
+        const int kb = q == 0 ? min(max(eb, 0), NCH - 1) : eb - da * q;
+        const uint16_t ua = src[ea + da * q], ub = src[kb];
+        Ra = (int8_t)(ua >> 8);
+        Ia = (int8_t)(ua & 0xFF);
+        Rb = (int8_t)(ub >> 8);
+        Ib = (int8_t)(ub & 0xFF);
+    });
     fft16k_t<+1>(v, lds, &ftw);   // element fft16k_tord(t) + 1024 q in v[q]
     float2 *dst = T + ((size_t)bpl * D + s) * NCH + fft16k_tord(t);
 #pragma unroll
@@ -456,25 +486,24 @@ k_trim_c2r_pk(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb,
     }
     __syncthreads();
     const int pbit = pol == 0 ? 4 : 0;
+    // channel ka = k (flipped: N - 1 - k) and its mirror kb = N - k (flipped:
+    // k - 1; k = 0 pairs with the zero Nyquist bin and reads nothing used): the
+    // staged words base +- 256 q, the same byte of each
+    const int da = flip ? -256 : 256;
+    const int ea = flip ? NCH - 1 - t : t, eb = flip ? t - 1 : NCH - t;
+    const int wa0 = ea >> 2, wb0 = eb >> 2;   // eb = -1 (t = 0, flipped): word 256 q - 1 for q >= 1
+    const uint32_t sha = 8 * (3 - (ea & 3)) + pbit, shb = 8 * (3 - (eb & 3)) + pbit;
     float2 v[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int k = t + 1024 * q;
-        const int km = (NCH - k) & (NCH - 1);
-        const int ka = flip ? NCH - 1 - k : k, kb = flip ? NCH - 1 - km : km;
-        const float kz = k == 0 ? 0.f : 1.f;
-        const uint32_t wa = stg[ka >> 2], wb = stg[kb >> 2];
-        const uint32_t sha = 8 * (3 - (ka & 3)) + pbit, shb = 8 * (3 - (kb & 3)) + pbit;
+    trim::c2r_inputs(tb.n32k, v, [&](int q, int &Ra, int &Ia, int &Rb, int &Ib) {
+        // only k = 0 (t = 0, q = 0) has its mirror outside the staged row
+        const int wb = q == 0 ? min(max(wb0, 0), NCH / 4 - 1) : wb0 - da * q;
+        const uint32_t a = stg[wa0 + da * q], b = stg[wb];
         // codes: im in the nibble's bits 0-1, re in bits 2-3; value = code - 2
-        const float ra = (float)((int)__builtin_amdgcn_ubfe(wa, sha + 2u, 2u) - 2);
-        const float ia = (float)((int)__builtin_amdgcn_ubfe(wa, sha, 2u) - 2);
-        const float rb = (float)((int)__builtin_amdgcn_ubfe(wb, shb + 2u, 2u) - 2);
-        const float ib = (float)((int)__builtin_amdgcn_ubfe(wb, shb, 2u) - 2);
-        const float2 a = make_float2(ra, kz * ia);
-        const float2 b = make_float2(kz * rb, kz * ib);
-        const float2 sum = cadd(a, cconj(b)), dif = csub(a, cconj(b));
-        v[q] = cadd(sum, mul_si<+1>(cmul(tw<+1>(tb.n32k, (uint32_t)k), dif)));
-    }
+        Ra = (int)__builtin_amdgcn_ubfe(a, sha + 2u, 2u) - 2;
+        Ia = (int)__builtin_amdgcn_ubfe(a, sha, 2u) - 2;
+        Rb = (int)__builtin_amdgcn_ubfe(b, shb + 2u, 2u) - 2;
+        Ib = (int)__builtin_amdgcn_ubfe(b, shb, 2u) - 2;
+    });
     __syncthreads();   // the staged words are read; the FFT reuses the array
     fft16k_t<+1>(v, lds, &ftw);   // element fft16k_tord(t) + 1024 q in v[q]
     float2 *dst = T + ((size_t)bpl * D + s) * NCH + fft16k_tord(t);

@@ -354,7 +354,7 @@ def unpack_codes(words):
 # Threshold policies.  REFERENCE: (th, off) of the first block-pol (block 0,
 # pol 0) latched for the whole stream and applied to both polarisations,
 # recomputed while th == 0 (vdif_inout_gpu_thread.cu:954-1005).
-# PER_BLOCK: each block-pol its own mean and ddof=1 sigma (north-star policy).
+# PER_BLOCK: each block-pol its own mean and ddof=1 sigma (per-block thresholds).
 # ---------------------------------------------------------------------------
 THRESH_REFERENCE = 0
 THRESH_PER_BLOCK = 1

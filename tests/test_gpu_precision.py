@@ -4,7 +4,7 @@ plus a strong tone (0 dB sigma) and a weak tone (-60 dB sigma), the 60 dB
 dynamic-range case.  Reported per block-pol: RMS and max |x - x64| / sigma64,
 and the number of 2-bit codes that differ from the oracle's.
 
-Bars (north star: RMS <= 1e-3 sigma); the max error of every mode is 50-70
+Bars (the project's accuracy target: RMS <= 1e-3 sigma vs FP64); the max error of every mode is 50-70
 times its RMS (the largest samples, at the decode-bias impulse that starts
 each snapshot, carry the largest absolute error), so the max bars are
 about 100 x the RMS bars:
@@ -12,7 +12,7 @@ about 100 x the RMS bars:
 * BF16X3:       RMS <= 1e-4 sigma, max <= 2e-3 sigma;
 * FP16:         RMS <= 1e-3 sigma, max <= 0.1 sigma;
 * BF16:         RMS <= 1e-2 sigma, max <= 1 sigma (one bf16 product per pass:
-                reported, outside the north-star bound).
+                reported, outside the 1e-3 sigma target).
 The weak tone must stand 10x above the error floor at its frequency.
 """
 import sys

@@ -3,7 +3,7 @@
 Bars:
 * unpack / reorder / zero fill / fid bitmap / counters: bit-exact;
 * resampled FP32 samples: RMS error <= 2e-6 sigma and max error <= 5e-5 sigma
-  of the FP64 oracle (the FP32 pipeline's rounding; north-star bound 1e-3 sigma);
+  of the FP64 oracle (the FP32 pipeline's rounding; the accuracy target is 1e-3 sigma);
 * 2-bit codes: a differing code is counted by the distance of its FP64 sample
   from the nearest decision boundary (-th, 0, +th after the offset) in three
   shells (<= 1e-6, 1e-6..1e-5, > 1e-5 sigma); none may lie beyond 1e-5 sigma

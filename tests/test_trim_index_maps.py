@@ -9,8 +9,9 @@ The kernels form those channel indices, and the packet kernel the staged word
 (channel / 4) and bit field (byte 3 - channel mod 4), as per-thread bases plus
 +-1024 q (words +-256 q), clamping only the mirror of k = 0, which pairs with
 the zero Nyquist bin and whose value is not used.  This restates that base-and-
-step form and checks it against the definition for every thread, q, flip and
-polarisation, including the flipped branch that no API call reaches today.
+step form and checks it against the definition for every thread, q and flip,
+including the flipped branch that no API call reaches today (the polarisation
+only adds a constant 0 or 4 to the bit position).
 Runs no GPU code."""
 import numpy as np
 

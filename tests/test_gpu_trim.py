@@ -69,13 +69,17 @@ def _oracle_block(beng_block, E, skip):
     return O.resample_trimmed(O.inflate(beng_block)[:, :NCH], E, skip)
 
 
-@pytest.mark.parametrize("frac", [6, 8, 10, 11])
-def test_trim_samples_vs_fp64(torch_cuda, frac):
+# skip 0 takes the segment spectrum from DC, where the R2C post-processing pairs
+# bin 0 with itself (Z[R/2 - 0] is Z[0]); 10/11 at SKIP_B (32 MHz) ends exactly
+# at the last bin R/2 (test_trim_codes_and_headers)
+@pytest.mark.parametrize("frac,skip_mhz", [(6, None), (8, None), (10, None), (11, None), (6, 0), (8, 0)])
+def test_trim_samples_vs_fp64(torch_cuda, frac, skip_mhz):
     torch = torch_cuda
-    D, E, R, P, skip = O.trim_geometry(frac, SKIP_A[frac])
+    skip_mhz = SKIP_A[frac] if skip_mhz is None else skip_mhz
+    D, E, R, P, skip = O.trim_geometry(frac, skip_mhz)
     n = 2
     beng = _spectra(600 + frac, D, n)
-    st = _stage(frac, max_spectra=n)
+    st = _stage(frac, skip=skip_mhz, max_spectra=n)
     _say("[trim] %d/11 resample_f32 n=%d (D=%d E=%d skip=%d)" % (frac, n, D, E, skip))
     x, stats = st.resample_f32(torch.from_numpy(beng).cuda())
     x = x.cpu().numpy().astype(np.float64)

@@ -1676,8 +1676,8 @@ int run_trim(sdbe_handle_t *h, const BpDesc *d_bpd, int n_bp, const OutDesc &od,
         const int nb = (n_bp - bp0) < h->chunk ? (n_bp - bp0) : h->chunk;
         const BpDesc *bpd = d_bpd + bp0;
         if (pk)
-            LAUNCH(h, KID_TRIM_C2R_PK, nb, s, kc2r_pk, dim3(D * nb), dim3(1024), bpd, h->V, h->tb, pk,
-                   (const int *)h->map);
+            LAUNCH(h, KID_TRIM_C2R_PK, nb, s, kc2r_pk, dim3(D * nb < h->n_cu ? D * nb : h->n_cu), dim3(1024), bpd,
+                   h->V, h->tb, pk, (const int *)h->map, D * nb);
         else
             LAUNCH(h, KID_TRIM_C2R, nb, s, kc2r, dim3(D * nb), dim3(1024), bpd, h->V, h->tb);
         if (f32)

@@ -470,63 +470,106 @@ k_trim_c2r(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb) {
 //   S & 7, word (c & 1023) >> 2: one 16-byte load per thread.
 // Both polarisations' codes share a byte (pol 0 in bits 4-7: re 6-7, im 4-5;
 // pol 1 in bits 0-3); a missing packet is staged as 0xAA bytes, which decode to
-// 0 + 0j.  Workgroups are taken in XCD-contiguous order, so the snapshots of a
-// block, whose words share cache lines, are read through one XCD's L2.
+// 0 + 0j.  The workgroups are persistent (one per CU) and take the snapshots in
+// XCD-contiguous order in each round, so the snapshots of a block, whose words
+// share cache lines, are read through one XCD's L2.
 template <int D, bool G2>
 __global__ void __launch_bounds__(1024)
 k_trim_c2r_pk(const BpDesc *__restrict__ bpd, float2 *__restrict__ T, Tables tb, const uint32_t *__restrict__ pk,
-              const int *__restrict__ map) {
+              const int *__restrict__ map, int n_snap) {
     __shared__ float lds[LDS_FFT_N];
     __shared__ FftTw ftw;
     fft_tw_load(&ftw, tb.fft);
-    const int L = xcd_contig((int)blockIdx.x, (int)gridDim.x);
-    const int bpl = L / D, s = L - bpl * D, t = threadIdx.x;
-    const int flip = bpd[bpl].flip, pol = bpd[bpl].pol;
-    const int S = D * bpd[bpl].blk + s, frame = S >> 7;
+    const int t = threadIdx.x, G = (int)gridDim.x;
     uint32_t *stg = reinterpret_cast<uint32_t *>(lds);   // NCH / 4 words
-    if constexpr (G2) {
-        const int cid = t >> 6;
-        const int p = map[(frame * G2_CIDS + cid) * G2_FIDS + ((S & 127) >> 3)];
-        uint4 v = make_uint4(0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu);
-        if (p != MAP_NONE)
-            v = *reinterpret_cast<const uint4 *>(pk + (size_t)p * G2_PKT_WORDS + 8 + 256 * (S & 7) + 4 * (t & 63));
-        reinterpret_cast<uint4 *>(stg)[t] = v;
-    } else {
+    // Persistent: the workgroup takes snapshots L = xcd_contig(wg) + G r (in each
+    // round the snapshots on one XCD are consecutive).  While a snapshot's
+    // transform runs, the packet words of the next one are in flight, from map
+    // entries read one snapshot earlier still.
+    auto lookup = [&](int L, int (&p)[2]) {   // packet index of each staged unit
+        const int bpl = L / D, S = D * bpd[bpl].blk + (L - bpl * D), frame = S >> 7;
+        if constexpr (G2) {
+            p[0] = map[(frame * G2_CIDS + (t >> 6)) * G2_FIDS + ((S & 127) >> 3)];
+        } else {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int g = t + 1024 * h;
-            const int cid = 2 * (g >> 4) + (g & 1), fid = (g >> 1) & 7;
-            const int p = map[(frame * 256 + cid) * 8 + fid];
-            uint2 v = make_uint2(0xAAAAAAAAu, 0xAAAAAAAAu);
-            if (p != MAP_NONE) v = *reinterpret_cast<const uint2 *>(pk + (size_t)p * PKT_WORDS + 8 + 2 * (S & 127));
-            reinterpret_cast<uint2 *>(stg)[g] = v;
+            for (int h = 0; h < 2; ++h) {
+                const int g = t + 1024 * h;
+                p[h] = map[(frame * 256 + 2 * (g >> 4) + (g & 1)) * 8 + ((g >> 1) & 7)];
+            }
         }
-    }
-    __syncthreads();
-    const int pbit = pol == 0 ? 4 : 0;
-    // channel ka = k (flipped: N - 1 - k) and its mirror kb = N - k (flipped:
-    // k - 1; k = 0 pairs with the zero Nyquist bin and reads nothing used): the
-    // staged words base +- 256 q, the same byte of each
-    const int da = flip ? -256 : 256;
-    const int ea = flip ? NCH - 1 - t : t, eb = flip ? t - 1 : NCH - t;
-    const int wa0 = ea >> 2, wb0 = eb >> 2;   // eb = -1 (t = 0, flipped): word 256 q - 1 for q >= 1
-    const uint32_t sha = 8 * (3 - (ea & 3)) + pbit, shb = 8 * (3 - (eb & 3)) + pbit;
-    float2 v[16];
-    trim::c2r_inputs(tb.n32k, v, [&](int q, int &Ra, int &Ia, int &Rb, int &Ib) {
-        // only k = 0 (t = 0, q = 0) has its mirror outside the staged row
-        const int wb = q == 0 ? min(max(wb0, 0), NCH / 4 - 1) : wb0 - da * q;
-        const uint32_t a = stg[wa0 + da * q], b = stg[wb];
-        // codes: im in the nibble's bits 0-1, re in bits 2-3; value = code - 2
-        Ra = (int)__builtin_amdgcn_ubfe(a, sha + 2u, 2u) - 2;
-        Ia = (int)__builtin_amdgcn_ubfe(a, sha, 2u) - 2;
-        Rb = (int)__builtin_amdgcn_ubfe(b, shb + 2u, 2u) - 2;
-        Ib = (int)__builtin_amdgcn_ubfe(b, shb, 2u) - 2;
-    });
-    __syncthreads();   // the staged words are read; the FFT reuses the array
-    fft16k_t<+1>(v, lds, &ftw);   // element fft16k_tord(t) + 1024 q in v[q]
-    float2 *dst = T + ((size_t)bpl * D + s) * NCH + fft16k_tord(t);
+    };
+    auto words = [&](int L, const int (&p)[2], uint4 &w) {   // the staged units (0xAA: missing)
+        const int bpl = L / D, S = D * bpd[bpl].blk + (L - bpl * D);
+        w = make_uint4(0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu, 0xAAAAAAAAu);
+        if constexpr (G2) {
+            if (p[0] != MAP_NONE)
+                w = *reinterpret_cast<const uint4 *>(pk + (size_t)p[0] * G2_PKT_WORDS + 8 + 256 * (S & 7) + 4 * (t & 63));
+        } else {
+            if (p[0] != MAP_NONE) {
+                const uint2 u = *reinterpret_cast<const uint2 *>(pk + (size_t)p[0] * PKT_WORDS + 8 + 2 * (S & 127));
+                w.x = u.x;
+                w.y = u.y;
+            }
+            if (p[1] != MAP_NONE) {
+                const uint2 u = *reinterpret_cast<const uint2 *>(pk + (size_t)p[1] * PKT_WORDS + 8 + 2 * (S & 127));
+                w.z = u.x;
+                w.w = u.y;
+            }
+        }
+    };
+    int L = xcd_contig((int)blockIdx.x, G);
+    int p[2] = {MAP_NONE, MAP_NONE}, pn[2] = {MAP_NONE, MAP_NONE};
+    uint4 w;
+    lookup(L, p);
+    words(L, p, w);
+    if (L + G < n_snap) lookup(L + G, pn);
+    for (;;) {   // every wave takes the same snapshots: the barriers below are uniform
+        const int bpl = L / D, s = L - bpl * D;
+        const int flip = bpd[bpl].flip, pol = bpd[bpl].pol;
+        // the staging area is the transform's work array: every wave must be done
+        // with the previous snapshot's transform (its last exchange is wave-local)
+        __syncthreads();
+        if constexpr (G2) {
+            reinterpret_cast<uint4 *>(stg)[t] = w;
+        } else {
+            reinterpret_cast<uint2 *>(stg)[t] = make_uint2(w.x, w.y);
+            reinterpret_cast<uint2 *>(stg)[t + 1024] = make_uint2(w.z, w.w);
+        }
+        __syncthreads();
+        const int pbit = pol == 0 ? 4 : 0;
+        // channel ka = k (flipped: N - 1 - k) and its mirror kb = N - k (flipped:
+        // k - 1; k = 0 pairs with the zero Nyquist bin and reads nothing used): the
+        // staged words base +- 256 q, the same byte of each
+        const int da = flip ? -256 : 256;
+        const int ea = flip ? NCH - 1 - t : t, eb = flip ? t - 1 : NCH - t;
+        const int wa0 = ea >> 2, wb0 = eb >> 2;   // eb = -1 (t = 0, flipped): word 256 q - 1 for q >= 1
+        const uint32_t sha = 8 * (3 - (ea & 3)) + pbit, shb = 8 * (3 - (eb & 3)) + pbit;
+        float2 v[16];
+        trim::c2r_inputs(tb.n32k, v, [&](int q, int &Ra, int &Ia, int &Rb, int &Ib) {
+            // only k = 0 (t = 0, q = 0) has its mirror outside the staged row
+            const int wb = q == 0 ? min(max(wb0, 0), NCH / 4 - 1) : wb0 - da * q;
+            const uint32_t a = stg[wa0 + da * q], b = stg[wb];
+            // codes: im in the nibble's bits 0-1, re in bits 2-3; value = code - 2
+            Ra = (int)__builtin_amdgcn_ubfe(a, sha + 2u, 2u) - 2;
+            Ia = (int)__builtin_amdgcn_ubfe(a, sha, 2u) - 2;
+            Rb = (int)__builtin_amdgcn_ubfe(b, shb + 2u, 2u) - 2;
+            Ib = (int)__builtin_amdgcn_ubfe(b, shb, 2u) - 2;
+        });
+        __syncthreads();   // the staged words are read; the FFT reuses the array
+        const int next = L + G;
+        if (next < n_snap) {   // workgroup-uniform
+            words(next, pn, w);
+            if (next + G < n_snap) lookup(next + G, pn);
+        }
+        // keep the next snapshot's loads here, ahead of the transform
+        __builtin_amdgcn_sched_barrier(0);
+        fft16k_t<+1>(v, lds, &ftw);   // element fft16k_tord(t) + 1024 q in v[q]
+        float2 *dst = T + ((size_t)bpl * D + s) * NCH + fft16k_tord(t);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
+        for (int q = 0; q < 16; ++q) dst[1024 * q] = v[q];
+        if (next >= n_snap) break;
+        L = next;
+    }
 }
 
 // SwarmR2C + R2dbeC2R of segment g of block bpl (workgroup = segment,

@@ -11,7 +11,9 @@ The kernels form those channel indices, and the packet kernel the staged word
 the zero Nyquist bin and whose value is not used.  This restates that base-and-
 step form and checks it against the definition for every thread, q and flip,
 including the flipped branch that no API call reaches today (the polarisation
-only adds a constant 0 or 4 to the bit position).
+only adds a constant 0 or 4 to the bit position).  It also checks the segment
+kernel's post-processing bases (k_trim_seg) against the R2C bin indices they
+replace, at both ends of the band.
 Runs no GPU code."""
 import numpy as np
 
@@ -64,3 +66,42 @@ def test_c2r_flip_reverses_channels():
             if t + 1024 * q:
                 assert kb1 == N - 1 - kb0
     assert np.array_equal(np.arange(N)[::-1][np.arange(N)], N - 1 - np.arange(N))
+
+
+# Segment post-processing of k_trim_seg (sdbe_trim.hpp): element j = t + NT q of
+# the C2R input needs X[k] and X[k'] of the R2C, k = skip + j, k' = skip + NP - j,
+# each from Z[k] and Z[(NR - k) mod NR] (Z[NR] read as Z[0]).  The kernel reads
+# them at per-thread bases plus non-negative multiples of NT:
+#   k = ka + NT q, NR - k = kb + NT (JT-1-q), k' = kc + NT (JT-1-q), NR - k' = kd + NT q,
+# with ka = skip + t, kc = skip + NT - t, kb = NR - NT (JT-1) - ka, kd = NR - NT (JT-1) - kc,
+# and maps an index equal to NR to 0 at q = 0 only.
+PRESETS = {6: (39, 32, 256), 8: (13, 8, 128), 10: (65, 64, 512), 11: (143, 128, 1024)}
+
+
+def test_segment_postprocessing_bases():
+    for frac, (D, E, NT) in PRESETS.items():
+        R, P = 256 * D, 256 * E
+        NR, NP, JT = R // 2, P // 2, P // 2 // NT
+        bin_hz = 4576e6 * frac / 11 / R
+        for skip in sorted({0, 1, NR - NP, (NR - NP) // 2, int(150e6 / bin_hz)}):
+            if skip + NP > NR:
+                continue
+            t = np.arange(NT)
+            ka, kc = skip + t, skip + NT - t
+            kb, kd = NR - NT * (JT - 1) - ka, NR - NT * (JT - 1) - kc
+            assert (kb >= 0).all() and (kd >= 0).all() and (kc >= 0).all()
+            for q in range(JT):
+                j = t + NT * q
+                k, k2 = skip + j, skip + NP - j
+                ia, ib = ka + NT * q, kb + NT * (JT - 1 - q)
+                ic, idd = kc + NT * (JT - 1 - q), kd + NT * q
+                if q == 0:
+                    ib = np.where(ib == NR, 0, ib)
+                    ic = np.where(ic == NR, 0, ic)
+                assert np.array_equal(ia, np.where(k == NR, 0, k))
+                assert np.array_equal(ib, (NR - k) % NR)
+                assert np.array_equal(ic, np.where(k2 == NR, 0, k2))
+                assert np.array_equal(idd, (NR - k2) % NR)
+                assert max(ia.max(), ib.max(), ic.max(), idd.max()) < NR
+                # the twiddle table xr has NR + 1 entries; k' = NR uses its last
+                assert k2.max() <= NR and k.max() < NR

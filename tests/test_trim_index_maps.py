@@ -105,3 +105,58 @@ def test_segment_postprocessing_bases():
                 assert max(ia.max(), ib.max(), ic.max(), idd.max()) < NR
                 # the twiddle table xr has NR + 1 entries; k' = NR uses its last
                 assert k2.max() <= NR and k.max() < NR
+
+
+# LDS addressing of the Stockham passes (sdbe_trim.hpp pass_reads / pass_writes):
+# the swizzle swz16(e) = e ^ ((e >> 4) & 15) and the decomposed forms the passes use
+# in its place, chosen by the same compile-time conditions, for every pass run.
+def _swz(e):
+    return e ^ ((e >> 4) & 15)
+
+
+def _swz_add(sb, c):
+    return (sb ^ ((c & 128) >> 4)) + c
+
+
+# (N, R, Ns, plain output) of each pass of every preset: the R2C's (its first pass
+# reads global memory and writes as Ns = 1; its last writes the plain layout), then
+# the C2R's (its last pass, None here, reads only)
+PASSES = [(4992, 16, 1, False), (4992, 8, 16, False), (4992, 13, 128, False), (4992, 3, 1664, True),
+          (1664, 16, 1, False), (1664, 8, 16, False), (1664, 13, 128, True),
+          (8320, 16, 1, False), (8320, 8, 16, False), (8320, 5, 128, False), (8320, 13, 640, True),
+          (18304, 16, 1, False), (18304, 8, 16, False), (18304, 11, 128, False), (18304, 13, 1408, True),
+          (4096, 16, 1, False), (4096, 16, 16, False), (4096, 16, 256, None),
+          (1024, 16, 1, False), (1024, 8, 16, False), (1024, 8, 128, None),
+          (8192, 16, 1, False), (8192, 16, 16, False), (8192, 16, 256, False), (8192, 2, 4096, None)]
+
+
+def test_pass_addressing_forms():
+    for N, R, NS, lin in PASSES:
+        NB = N // R
+        b = np.arange(NB)
+        for i in range(R):   # the reads (the per-access branch compares swz16 with itself)
+            e = b + NB * i
+            got = _swz_add(_swz(b), NB * i) if NB % 128 == 0 else _swz(e)
+            assert np.array_equal(got, _swz(e)), (N, R, NS, i)
+        if lin is None:
+            continue   # pass_to_regs: no LDS writes
+        base = (b // NS) * NS * R + (b % NS)
+        sb = _swz(base)
+        for k in range(R):
+            e = base + NS * k
+            if lin:
+                got, want = e, e
+            elif NS % 128 == 0:
+                got, want = _swz_add(sb, NS * k), _swz(e)
+            elif NS == 16 and R & (R - 1) == 0:
+                got, want = (sb ^ k) + 16 * k, _swz(e)
+            elif NS == 1 and R == 16:
+                got, want = sb ^ k, _swz(e)
+            else:
+                got, want = _swz(e), _swz(e)
+            assert np.array_equal(got, want), (N, R, NS, k)
+            assert got.max() < N and got.min() >= 0
+        # each pass's writes are a permutation of the N elements
+        if not lin:
+            allw = np.concatenate([_swz(base + NS * k) for k in range(R)])
+            assert np.array_equal(np.sort(allw), np.arange(N))

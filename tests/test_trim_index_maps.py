@@ -13,7 +13,8 @@ step form and checks it against the definition for every thread, q and flip,
 including the flipped branch that no API call reaches today (the polarisation
 only adds a constant 0 or 4 to the bit position).  It also checks the segment
 kernel's post-processing bases (k_trim_seg) against the R2C bin indices they
-replace, at both ends of the band.
+replace, at both ends of the band, and the base-plus-offset forms of the Stockham
+passes' swizzled LDS addresses against the swizzle itself, for every pass run.
 Runs no GPU code."""
 import numpy as np
 
